@@ -1,0 +1,96 @@
+/*
+ * KIFMM3d CPU oracle -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C, single-threaded restatement of the reference's CPU evaluate path
+ * (aashays/fmm-g, fmm3d_mpi/) on one rank.  Only tests/, the smoke check and the
+ * bench's cpu_baseline leg may load it, and only as the checker.  The product
+ * (fmm-g_amd/) never links or calls it.
+ *
+ * Layout conventions follow the reference:
+ *   positions / normals : xyz interleaved, 3 doubles per point
+ *   densities           : srcDOF per point, interleaved
+ *   values              : trgDOF per point, interleaved
+ *   dense matrices      : column-major (common/nummat.hpp:97-105)
+ */
+#ifndef KIFMM_ORACLE_H
+#define KIFMM_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* surface / operator identifiers (matmgnt3d_mpi.hpp enum UE,UC,DE,DC) */
+enum { ORC_UE = 0, ORC_UC = 1, ORC_DE = 2, ORC_DC = 3 };
+
+/* list tags (let3d_mpi.hpp:21-33) */
+enum { ORC_CBTR = 1, ORC_USER = 2, ORC_EVTR = 4, ORC_OWNR = 8, ORC_SRCE = 16 };
+
+int  orc_src_dof(int kt);
+int  orc_trg_dof(int kt);
+int  orc_mm_kernel(int kt);
+int  orc_hom_deg(int kt, double *deg);
+int  orc_kernel_matrix(int kt, const double *coefs, int ns, const double *spos,
+                       const double *snor, int nt, const double *tpos, double *mat);
+
+int  orc_sample_count(int np);
+void orc_sample_pos(int np, double R, double *pos);
+void orc_regular_pos(int np, double R, double *pos);
+
+void orc_rfft3(int n, int howmany, const double *in, double *out);
+void orc_irfft3(int n, int howmany, const double *in, double *out);
+
+/* tree ------------------------------------------------------------------ */
+typedef struct {
+  int n_nodes;
+  int max_level;
+  int *parent, *child, *depth, *path, *tag;
+  int *src_cnt, *trg_cnt;          /* points in the subtree */
+  int *src_beg, *src_num;          /* glbSrcExaBeg / glbSrcExaNum (leaves, -1 otherwise) */
+  int *trg_beg, *trg_num;          /* evaTrgExaBeg / evaTrgExaNum (EVTR leaves, -1 otherwise) */
+  int *src_perm, *trg_perm;        /* packed slot -> caller's point index */
+  int64_t *u_off, *v_off, *w_off, *x_off;
+  int *u_lst, *v_lst, *w_lst, *x_lst;
+  int n_src, n_trg;
+} orc_tree;
+
+int  orc_tree_build(orc_tree *t, int n_src, const double *spos, int n_trg,
+                    const double *tpos, const double ctr[3], int root_level,
+                    int pts_max, int max_level);
+void orc_tree_free(orc_tree *t);
+
+/* operators ----------------------------------------------------------- */
+typedef struct {
+  int kt_mm, np, sd, td;
+  double coefs[4];
+  int ncoefs;
+  int n_ue, n_uc, n_de, n_dc;
+  double *pinv_up;   /* (n_ue*sd) x (n_uc*td), column-major */
+  double *pinv_dn;   /* (n_de*sd) x (n_dc*td) */
+  double *m2m;       /* 8 x [(n_uc*td) x (n_ue*sd)], child index 4a+2b+c */
+  double *l2l;       /* 8 x [(n_dc*td) x (n_de*sd)] */
+  double *m2l;       /* 343 slots (offset+3 in each axis), each (sd*td) x H complex interleaved */
+  double deg[4];
+} orc_ops;
+
+int  orc_ops_init(orc_ops *op, int kt_mm, const double *coefs, int ncoefs, int np,
+                  const double *pinv_up, const double *pinv_dn);
+void orc_ops_free(orc_ops *op);
+int  orc_check_matrix(int kt_mm, const double *coefs, int ncoefs, int np, int which,
+                      double *mat);
+
+/* evaluate ------------------------------------------------------------- */
+int  orc_evaluate(const orc_tree *t, const orc_ops *op, int kt_user,
+                  const double *coefs, int root_level, const double ctr[3],
+                  const double *spos, const double *snor, const double *tpos,
+                  const double *src_den, double *trg_val, double *ue_out,
+                  double *de_out);
+
+int  orc_direct(int kt, const double *coefs, int ns, const double *spos,
+                const double *snor, const double *sden, int nt,
+                const double *tpos, double *tval, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
