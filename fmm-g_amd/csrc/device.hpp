@@ -1,0 +1,84 @@
+// Host-side launchers for the HIP kernels of the evaluate pass.  All pointers are
+// device pointers; every launcher enqueues on `stream` and returns the launch
+// status.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernel3d.hpp"
+
+namespace kifmm {
+
+// A chunk of at most kNearChunk targets of one leaf and its source items.
+struct NearChunk {
+  int node, tbeg, tnum, ibeg, iend;
+};
+constexpr int kNearChunk = 128;
+// Source item code: (node << 2) | kind
+enum ItemKind { ITEM_EXACT = 0, ITEM_UE = 1, ITEM_DE = 2 };
+
+// A to-surface task: check surface of `node` receives the field of the exact
+// sources of the leaves items[ibeg..iend).
+struct SurfTask {
+  int node, ibeg, iend;
+};
+
+struct NearArgs {
+  const NearChunk* chunks; int nchunks;
+  const int* items;
+  const double* spos; const double* snor; const double* sden;
+  const int* src_beg; const int* src_num;
+  const double4* geom;
+  const double* sam_ue; int n_ue;
+  const double* sam_de; int n_de;
+  const double* UE; int sz_ue;
+  const double* DE; int sz_de;
+  const double* tpos; double* tval;
+};
+
+struct SurfArgs {
+  const SurfTask* tasks; int ntasks;
+  const int* items;
+  const double* spos; const double* snor; const double* sden;
+  const int* src_beg; const int* src_num;
+  const double4* geom;
+  const double* sam; int n_sam;
+  double* out; int sz_out;
+};
+
+struct GemmArgs {
+  int M, K; const double* A;
+  const double* B; const int* bidx; int64_t ldb; const double* kscale; int ksd;
+  double* C; const int* cidx; int64_t ldc; const double* rscale; int rsd;
+  int ncol; int accumulate;
+};
+
+struct VlistArgs {
+  int np, n, H, sd, td;
+  int nsrc, ntrg, ngrp;
+  const int* src_node; const int* trg_node; const int* depth; int root_level;
+  const double* deg;           // per source DOF
+  const int* shell;            // surface sample -> block-relative (k*np+j)*np+i
+  const double2* twiddle;      // n entries exp(-2 pi i m/n)
+  const double* UE; int sz_ue;
+  double* DC; int sz_dc;
+  const int* halo; const uint64_t* mask; const int* tgt;
+  const double2* m2l;          // [343][sd*td][H]
+  double2* dspec;              // [nsrc][sd][H]
+  double2* espec;              // [ntrg][td][H]
+};
+
+hipError_t launch_gather(const double* in, const int* perm, double* out, int64_t n, int dof, hipStream_t s);
+hipError_t launch_scatter(const double* in, const int* perm, double* out, int64_t n, int dof, hipStream_t s);
+hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s);
+hipError_t launch_near(int kt, const NearArgs& a, const KParams<double>& pu, const KParams<double>& pm,
+                       hipStream_t s);
+hipError_t launch_surface(int kt, const SurfArgs& a, const KParams<double>& pu, hipStream_t s);
+hipError_t launch_gemm(const GemmArgs& g, hipStream_t s);
+hipError_t launch_vlist(const VlistArgs& v, hipStream_t s);
+hipError_t launch_direct(int kt, const KParams<double>& p, int64_t ns, const double* spos,
+                         const double* snor, const double* sden, int nt, const double* tpos,
+                         double* tval, hipStream_t s);
+
+}  // namespace kifmm

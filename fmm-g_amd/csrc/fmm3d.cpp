@@ -1,0 +1,485 @@
+// FMM3d setup and evaluate (see fmm3d.hpp).  Citations are to the reference's
+// fmm3d_mpi/ sources.
+#include "fmm3d.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+
+#include "kernel3d.hpp"
+
+namespace kifmm {
+
+const char* const kPhaseNames[PH_COUNT] = {"EvlIni", "EvlUpwComp", "EvlVList", "EvlXList",
+                                           "EvlCombine", "EvlUList", "EvlFinalize"};
+
+DeviceBuffer::~DeviceBuffer() { release(); }
+
+hipError_t DeviceBuffer::alloc(size_t bytes) {
+  release();
+  if (bytes == 0) return hipSuccess;
+  hipError_t e = hipMalloc(&ptr_, bytes);
+  if (e == hipSuccess) bytes_ = bytes;
+  else ptr_ = nullptr;
+  return e;
+}
+
+hipError_t DeviceBuffer::upload(const void* host, size_t bytes) {
+  hipError_t e = alloc(bytes);
+  if (e != hipSuccess || bytes == 0) return e;
+  return hipMemcpy(ptr_, host, bytes, hipMemcpyHostToDevice);
+}
+
+void DeviceBuffer::release() {
+  if (ptr_) (void)hipFree(ptr_);
+  ptr_ = nullptr;
+  bytes_ = 0;
+}
+
+FMM3d::~FMM3d() {
+  for (auto ev : events_) (void)hipEventDestroy(ev);
+}
+
+int FMM3d::srcDOF() const { return src_dof(kt_); }
+int FMM3d::trgDOF() const { return trg_dof(kt_); }
+
+int FMM3d::check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return 0;
+  err_ = std::string(what) + ": " + hipGetErrorString(e);
+  return -5;
+}
+
+int FMM3d::setPoints(int64_t ns, const double* spos, const double* snor, int64_t nt, const double* tpos) {
+  if (ns < 0 || nt < 0) return fail(-2, "negative point count");
+  if ((ns > 0 && !spos) || (nt > 0 && !tpos)) return fail(-2, "null point array");
+  ns_ = ns; nt_ = nt;
+  spos_.assign(spos, spos + 3 * ns);
+  if (snor) snor_.assign(snor, snor + 3 * ns);
+  else snor_.assign(3 * ns, 0.0);
+  tpos_.assign(tpos, tpos + 3 * nt);
+  setup_done_ = false;
+  host_ready_ = false;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// setup: fmm3d_setup_mpi.cpp:41-565 (single rank)
+// ---------------------------------------------------------------------------
+int FMM3d::setup() {
+  int rc = setupHost();
+  if (rc) return rc;
+  rc = upload_state();
+  if (rc) return rc;
+  setup_done_ = true;
+  return 0;
+}
+
+int FMM3d::setupHost() {
+  setup_done_ = false;
+  host_ready_ = false;
+  if (!kernel_known(kt_)) return fail(-2, "unknown kernel type " + std::to_string(kt_));
+  const int km = mm_kernel(kt_);
+  if (km < 0) return fail(-2, "kernel " + std::to_string(kt_) + " has no equivalent-density kernel");
+  if ((kt_ == 511 || kt_ == 521) && coefs_.size() < 2) return fail(-2, "Navier kernels need (mu, nu)");
+  if (coefs_.empty()) return fail(-2, "kernel coefficients missing");
+  if (np_ < 3 || np_ > 8) return fail(-2, "np must lie in [3,8]");
+  if (precision_ != PREC_FP64 && precision_ != PREC_MIXED) return fail(-2, "unknown precision mode");
+  int rc = build_tree(spos_.data(), ns_, tpos_.data(), nt_, ctr_, root_level_, pts_max_, max_level_, tree_, &err_);
+  if (rc) return rc;
+  std::string e;
+  std::vector<double> mmc = coefs_;
+  if (kt_ == 312 || kt_ == 322) mmc.clear();  // fmm3d_setup_mpi.cpp:119,121
+  if (mmc.empty()) mmc = {1.0, 0.25};
+  ops_ = get_operators(km, mmc, np_, &e);
+  if (!ops_) return fail(-2, e);
+  rc = build_tasks();
+  if (rc) return rc;
+  host_ready_ = true;
+  return 0;
+}
+
+int FMM3d::build_tasks() {
+  const Tree& t = tree_;
+  const Operators& op = *ops_;
+  const int N = t.n_nodes, SD = srcDOF(), TD = trgDOF(), sd = op.sd;
+  const int szUE = op.dat(UE), szDC = op.dat(DC);
+  work_ = Work();
+
+  // --- near field: U, W (S2T or M2T), X (S2T fallback), L2T ---
+  near_chunks_h_.clear(); near_items_h_.clear();
+  for (int B = 0; B < N; B++) {
+    if (!(t.tag[B] & TAG_EVTR) || !t.leaf(B)) continue;
+    const int ibeg = int(near_items_h_.size());
+    const int tn = t.trg_num[B];
+    double exact = 0, surf = 0;
+    for (int64_t q = t.off[LIST_U][B]; q < t.off[LIST_U][B + 1]; q++) {
+      const int A = t.lst[LIST_U][q];
+      near_items_h_.push_back(A << 2 | ITEM_EXACT);
+      exact += t.src_num[A];
+    }
+    for (int64_t q = t.off[LIST_W][B]; q < t.off[LIST_W][B + 1]; q++) {  // :1146-1165
+      const int A = t.lst[LIST_W][q];
+      if (t.src_beg[A] >= 0 && t.src_num[A] * SD < szUE) {
+        near_items_h_.push_back(A << 2 | ITEM_EXACT);
+        exact += t.src_num[A];
+      } else {
+        near_items_h_.push_back(A << 2 | ITEM_UE);
+        surf += op.n_sam[UE];
+      }
+    }
+    if (tn * TD < szDC)  // :1249
+      for (int64_t q = t.off[LIST_X][B]; q < t.off[LIST_X][B + 1]; q++) {
+        const int A = t.lst[LIST_X][q];
+        near_items_h_.push_back(A << 2 | ITEM_EXACT);
+        exact += t.src_num[A];
+      }
+    if (t.depth[B] >= 2) {  // DE is zero above depth 2 (:1378)
+      near_items_h_.push_back(B << 2 | ITEM_DE);
+      surf += op.n_sam[DE];
+    }
+    const int iend = int(near_items_h_.size());
+    for (int c = 0; c < tn; c += kNearChunk)
+      near_chunks_h_.push_back(NearChunk{B, t.trg_beg[B] + c, std::min(kNearChunk, tn - c), ibeg, iend});
+    work_.near_exact_pairs += double(tn) * exact;
+    work_.near_pairs += double(tn) * (exact + surf);
+  }
+
+  // --- S2M (:557-577) and X-list S2L (:1241-1257) ---
+  s2m_tasks_h_.clear(); s2l_tasks_h_.clear(); surf_items_h_.clear();
+  for (int B = 1; B < N; B++)
+    if ((t.tag[B] & TAG_CBTR) && t.leaf(B)) {
+      int ib = int(surf_items_h_.size());
+      surf_items_h_.push_back(B);
+      s2m_tasks_h_.push_back(SurfTask{B, ib, ib + 1});
+      work_.s2m_pairs += double(t.src_num[B]) * op.n_sam[UC];
+    }
+  for (int B = 0; B < N; B++) {
+    if (!(t.tag[B] & TAG_EVTR)) continue;
+    if (t.off[LIST_X][B] == t.off[LIST_X][B + 1]) continue;
+    if (t.leaf(B) && t.trg_num[B] * TD < szDC) continue;
+    int ib = int(surf_items_h_.size());
+    for (int64_t q = t.off[LIST_X][B]; q < t.off[LIST_X][B + 1]; q++) {
+      surf_items_h_.push_back(t.lst[LIST_X][q]);
+      work_.s2l_pairs += double(t.src_num[t.lst[LIST_X][q]]) * op.n_sam[DC];
+    }
+    s2l_tasks_h_.push_back(SurfTask{B, ib, int(surf_items_h_.size())});
+  }
+
+  // --- level-batched GEMM steps ---
+  up_steps_.clear(); dn_steps_.clear(); lvl_idx_h_.clear(); scales_h_.clear();
+  auto scale_vec = [&](int lvl, int sign) {
+    int off = int(scales_h_.size());
+    for (int s = 0; s < sd; s++) scales_h_.push_back(std::ldexp(1.0, sign * int(lvl * op.deg[s])));
+    return off;
+  };
+  auto add_step = [&](std::vector<GemmStep>& v, int mat, int src, int dst, const std::vector<int>& b,
+                      const std::vector<int>& c, int ks, int rs, int acc) {
+    if (b.empty()) return;
+    GemmStep st{mat, src, dst, int(b.size()), acc, int64_t(lvl_idx_h_.size()), ks, rs};
+    lvl_idx_h_.insert(lvl_idx_h_.end(), b.begin(), b.end());
+    lvl_idx_h_.insert(lvl_idx_h_.end(), c.begin(), c.end());
+    v.push_back(st);
+  };
+  const int rl = t.root_level;
+  std::vector<std::vector<int>> by_depth(t.max_depth + 1);
+  for (int i = 0; i < N; i++) by_depth[t.depth[i]].push_back(i);
+  for (int d = t.max_depth; d >= 1; d--) {  // :557-595, children before parents
+    std::vector<int> mb, mc, pb;
+    for (int i : by_depth[d]) {
+      if (!(t.tag[i] & TAG_CBTR)) continue;
+      pb.push_back(i);
+      if (!t.leaf(i)) { mb.push_back(t.child[i]); mc.push_back(i); }
+    }
+    work_.m2m_cols += double(mc.size());
+    work_.up_cols += double(pb.size());
+    add_step(up_steps_, 0, 0, 1, mb, mc, scale_vec(d + 1 + rl, +1), -1, 0);
+    add_step(up_steps_, 1, 1, 0, pb, pb, -1, scale_vec(d + rl, -1), 0);
+  }
+  for (int d = 2; d <= t.max_depth; d++) {  // :1368-1382, parents before children
+    std::vector<int> pb;
+    std::vector<int> lb[8], lc[8];
+    for (int i : by_depth[d]) {
+      if (!(t.tag[i] & TAG_EVTR)) continue;
+      pb.push_back(i);
+      if (d >= 3) {
+        int p = t.parent[i], c = i - t.child[p];
+        lb[c].push_back(p); lc[c].push_back(i);
+      }
+    }
+    if (d >= 3) {
+      int ks = scale_vec(d - 1 + rl, +1);
+      for (int c = 0; c < 8; c++) {
+        work_.l2l_cols += double(lc[c].size());
+        add_step(dn_steps_, 3 + c, 2, 3, lb[c], lc[c], ks, -1, 1);
+      }
+    }
+    work_.dn_cols += double(pb.size());
+    add_step(dn_steps_, 2, 3, 2, pb, pb, -1, scale_vec(d + rl, -1), 0);
+  }
+
+  // --- V list as sibling groups ---
+  vsrc_node_h_.clear(); vtrg_node_h_.clear(); vhalo_h_.clear(); vtgt_h_.clear(); vmask_h_.clear();
+  std::vector<int> src_slot(N, -1);
+  std::map<int, int> grp;
+  for (int B = 0; B < N; B++) {
+    if (!(t.tag[B] & TAG_EVTR)) continue;
+    const int64_t b0 = t.off[LIST_V][B], b1 = t.off[LIST_V][B + 1];
+    if (b0 == b1) continue;
+    const int P = t.parent[B], tau = B - t.child[P];
+    auto it = grp.find(P);
+    int g;
+    if (it == grp.end()) {
+      g = int(grp.size());
+      grp[P] = g;
+      vhalo_h_.resize(size_t(g + 1) * 216, -1);
+      vtgt_h_.resize(size_t(g + 1) * 8, -1);
+      vmask_h_.resize(size_t(g + 1) * 32, 0);
+    } else g = it->second;
+    vtgt_h_[size_t(g) * 8 + tau] = int(vtrg_node_h_.size());
+    vtrg_node_h_.push_back(B);
+    for (int64_t q = b0; q < b1; q++) {
+      const int A = t.lst[LIST_V][q];
+      if (src_slot[A] < 0) { src_slot[A] = int(vsrc_node_h_.size()); vsrc_node_h_.push_back(A); }
+      int r = 0;
+      for (int d = 0; d < 3; d++) {
+        int rel = t.path[3 * A + d] - 2 * t.path[3 * P + d] + 2;
+        if (rel < 0 || rel > 5) return fail(-6, "V-list member outside the parent's colleague halo");
+        r = r * 6 + rel;
+      }
+      int& h = vhalo_h_[size_t(g) * 216 + r];
+      if (h >= 0 && h != src_slot[A]) return fail(-6, "inconsistent V-list halo");
+      h = src_slot[A];
+      vmask_h_[size_t(g) * 32 + tau * 4 + (r >> 6)] |= (uint64_t(1) << (r & 63));
+      work_.vlist_pairs += 1;
+    }
+  }
+  work_.n_vsrc = double(vsrc_node_h_.size());
+  work_.n_vtrg = double(vtrg_node_h_.size());
+  work_.n_vgrp = double(grp.size());
+  return 0;
+}
+
+int FMM3d::upload_state() {
+  const Tree& t = tree_;
+  const Operators& op = *ops_;
+  const int N = t.n_nodes, SD = srcDOF(), TD = trgDOF();
+  int rc;
+#define KIFMM_UP(buf, vec, what) \
+  if ((rc = check((buf).upload((vec).data(), sizeof((vec)[0]) * (vec).size()), what))) return rc;
+  // packed points (leaf order)
+  std::vector<double> sp(3 * ns_), sn(3 * ns_), tp(3 * nt_);
+  for (int64_t k = 0; k < ns_; k++) {
+    int64_t o = t.src_perm[k];
+    for (int d = 0; d < 3; d++) { sp[3 * k + d] = spos_[3 * o + d]; sn[3 * k + d] = snor_[3 * o + d]; }
+  }
+  for (int64_t k = 0; k < nt_; k++) {
+    int64_t o = t.trg_perm[k];
+    for (int d = 0; d < 3; d++) tp[3 * k + d] = tpos_[3 * o + d];
+  }
+  KIFMM_UP(spos_d_, sp, "upload sources");
+  KIFMM_UP(snor_d_, sn, "upload normals");
+  KIFMM_UP(tpos_d_, tp, "upload targets");
+  KIFMM_UP(sperm_d_, t.src_perm, "upload source permutation");
+  KIFMM_UP(tperm_d_, t.trg_perm, "upload target permutation");
+  if ((rc = check(sden_d_.alloc(sizeof(double) * std::max<int64_t>(1, ns_ * SD)), "alloc densities"))) return rc;
+  if ((rc = check(tval_d_.alloc(sizeof(double) * std::max<int64_t>(1, nt_ * TD)), "alloc values"))) return rc;
+  // node geometry
+  std::vector<double4> geom(N);
+  for (int i = 0; i < N; i++) {
+    double c[3];
+    t.center(i, c);
+    geom[i] = make_double4(c[0], c[1], c[2], t.radius(i));
+  }
+  KIFMM_UP(geom_d_, geom, "upload geometry");
+  KIFMM_UP(depth_d_, t.depth, "upload depths");
+  KIFMM_UP(sbeg_d_, t.src_beg, "upload ranges");
+  KIFMM_UP(snum_d_, t.src_num, "upload ranges");
+  // per-node equivalent / check buffers
+  const size_t n = size_t(N);
+  if ((rc = check(ue_d_.alloc(sizeof(double) * n * op.dat(UE)), "alloc UE"))) return rc;
+  if ((rc = check(uc_d_.alloc(sizeof(double) * n * op.dat(UC)), "alloc UC"))) return rc;
+  if ((rc = check(de_d_.alloc(sizeof(double) * n * op.dat(DE)), "alloc DE"))) return rc;
+  if ((rc = check(dc_d_.alloc(sizeof(double) * n * op.dat(DC)), "alloc DC"))) return rc;
+  // tasks
+  KIFMM_UP(near_chunks_d_, near_chunks_h_, "upload near chunks");
+  KIFMM_UP(near_items_d_, near_items_h_, "upload near items");
+  KIFMM_UP(s2m_tasks_d_, s2m_tasks_h_, "upload S2M tasks");
+  KIFMM_UP(s2l_tasks_d_, s2l_tasks_h_, "upload S2L tasks");
+  KIFMM_UP(surf_items_d_, surf_items_h_, "upload surface items");
+  KIFMM_UP(lvl_idx_d_, lvl_idx_h_, "upload level columns");
+  KIFMM_UP(scales_d_, scales_h_, "upload level scales");
+  std::vector<double> ones(8, 1.0);
+  KIFMM_UP(ones_d_, ones, "upload ones");
+  // operators
+  for (int s = 0; s < 4; s++) KIFMM_UP(sam_d_[s], op.sam[s], "upload surfaces");
+  {
+    std::vector<int> shell_blk(op.shell.size());
+    const int np = op.np, nn = op.n, o = np / 2;
+    for (size_t q = 0; q < op.shell.size(); q++) {
+      int g = op.shell[q];
+      int c = g % nn - o, b = (g / nn) % nn - o, a = g / (nn * nn) - o;
+      shell_blk[q] = (a * np + b) * np + c;
+    }
+    KIFMM_UP(shell_d_, shell_blk, "upload shell");
+    std::vector<double2> tw(nn);
+    for (int m = 0; m < nn; m++) tw[m] = make_double2(std::cos(2.0 * M_PI * m / nn), -std::sin(2.0 * M_PI * m / nn));
+    KIFMM_UP(twiddle_d_, tw, "upload twiddles");
+    std::vector<double> deg(op.deg, op.deg + op.sd);
+    KIFMM_UP(deg_d_, deg, "upload degrees");
+  }
+  KIFMM_UP(pup_d_, op.pinv_up, "upload pinv up");
+  KIFMM_UP(pdn_d_, op.pinv_dn, "upload pinv down");
+  KIFMM_UP(m2m_d_, op.m2m, "upload M2M");
+  KIFMM_UP(l2l_d_, op.l2l, "upload L2L");
+  KIFMM_UP(m2l_d_, op.m2l, "upload M2L");
+  // V list
+  KIFMM_UP(vsrc_d_, vsrc_node_h_, "upload V sources");
+  KIFMM_UP(vtrg_d_, vtrg_node_h_, "upload V targets");
+  KIFMM_UP(vhalo_d_, vhalo_h_, "upload V halo");
+  KIFMM_UP(vtgt_d_, vtgt_h_, "upload V groups");
+  KIFMM_UP(vmask_d_, vmask_h_, "upload V masks");
+  if ((rc = check(dspec_d_.alloc(sizeof(double2) * vsrc_node_h_.size() * op.sd * op.H), "alloc V spectra"))) return rc;
+  if ((rc = check(espec_d_.alloc(sizeof(double2) * vtrg_node_h_.size() * op.td * op.H), "alloc V spectra"))) return rc;
+#undef KIFMM_UP
+  if (events_.empty()) {
+    events_.resize(PH_COUNT + 1);
+    for (auto& ev : events_)
+      if ((rc = check(hipEventCreate(&ev), "create events"))) return rc;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// evaluate: fmm3d_eval_mpi.cpp:396-1515
+// ---------------------------------------------------------------------------
+int FMM3d::evaluate(const double* d_src_den, double* d_trg_val, hipStream_t s) {
+  if (!setup_done_) return fail(-3, "evaluate called before setup");
+  if ((ns_ > 0 && !d_src_den) || (nt_ > 0 && !d_trg_val)) return fail(-2, "null density or value pointer");
+  const Tree& t = tree_;
+  const Operators& op = *ops_;
+  const int N = t.n_nodes, SD = srcDOF(), TD = trgDOF();
+  const KParams<double> pu = make_params(kt_, coefs_.data());
+  const KParams<double> pm = make_params(op.kt_mm, op.coefs.data());
+  int rc;
+  auto mark = [&](int ph) { if (timing_) (void)hipEventRecord(events_[ph], s); };
+
+  mark(PH_INI);
+  // zero the work vectors (:425-438)
+  if ((rc = check(launch_fill(ue_d_.as<double>(), int64_t(N) * op.dat(UE), 0.0, s), "zero UE"))) return rc;
+  if ((rc = check(launch_fill(uc_d_.as<double>(), int64_t(N) * op.dat(UC), 0.0, s), "zero UC"))) return rc;
+  if ((rc = check(launch_fill(de_d_.as<double>(), int64_t(N) * op.dat(DE), 0.0, s), "zero DE"))) return rc;
+  if ((rc = check(launch_fill(dc_d_.as<double>(), int64_t(N) * op.dat(DC), 0.0, s), "zero DC"))) return rc;
+  if ((rc = check(launch_gather(d_src_den, sperm_d_.as<int>(), sden_d_.as<double>(), ns_, SD, s), "gather"))) return rc;
+
+  SurfArgs sa{};
+  sa.items = surf_items_d_.as<int>();
+  sa.spos = spos_d_.as<double>(); sa.snor = snor_d_.as<double>(); sa.sden = sden_d_.as<double>();
+  sa.src_beg = sbeg_d_.as<int>(); sa.src_num = snum_d_.as<int>();
+  sa.geom = geom_d_.as<double4>();
+
+  double* bufs[4] = {ue_d_.as<double>(), uc_d_.as<double>(), de_d_.as<double>(), dc_d_.as<double>()};
+  const int szs[4] = {op.dat(UE), op.dat(UC), op.dat(DE), op.dat(DC)};
+  auto run_steps = [&](const std::vector<GemmStep>& steps) -> int {
+    for (const GemmStep& st : steps) {
+      GemmArgs g{};
+      switch (st.mat) {
+        case 0: g.M = op.dat(UC); g.K = 8 * op.dat(UE); g.A = m2m_d_.as<double>(); break;
+        case 1: g.M = op.dat(UE); g.K = op.dat(UC); g.A = pup_d_.as<double>(); break;
+        case 2: g.M = op.dat(DE); g.K = op.dat(DC); g.A = pdn_d_.as<double>(); break;
+        default:
+          g.M = op.dat(DC); g.K = op.dat(DE);
+          g.A = l2l_d_.as<double>() + size_t(st.mat - 3) * op.dat(DC) * op.dat(DE);
+      }
+      g.B = bufs[st.src]; g.ldb = szs[st.src];
+      g.C = bufs[st.dst]; g.ldc = szs[st.dst];
+      g.bidx = lvl_idx_d_.as<int>() + st.idx_off;
+      g.cidx = lvl_idx_d_.as<int>() + st.idx_off + st.ncol;
+      if (st.ks_off >= 0) { g.kscale = scales_d_.as<double>() + st.ks_off; g.ksd = op.sd; }
+      else { g.kscale = ones_d_.as<double>(); g.ksd = 1; }
+      if (st.rs_off >= 0) { g.rscale = scales_d_.as<double>() + st.rs_off; g.rsd = op.sd; }
+      else { g.rscale = ones_d_.as<double>(); g.rsd = 1; }
+      g.ncol = st.ncol; g.accumulate = st.accumulate;
+      int r = check(launch_gemm(g, s), "level GEMM");
+      if (r) return r;
+    }
+    return 0;
+  };
+
+  // up pass (:480-619)
+  mark(PH_UPW);
+  sa.tasks = s2m_tasks_d_.as<SurfTask>(); sa.ntasks = int(s2m_tasks_h_.size());
+  sa.sam = sam_d_[UC].as<double>(); sa.n_sam = op.n_sam[UC];
+  sa.out = uc_d_.as<double>(); sa.sz_out = op.dat(UC);
+  if ((rc = check(launch_surface(kt_, sa, pu, s), "S2M"))) return rc;
+  if ((rc = run_steps(up_steps_))) return rc;
+
+  // V list (:1007-1059)
+  mark(PH_VLIST);
+  VlistArgs v{};
+  v.np = op.np; v.n = op.n; v.H = op.H; v.sd = op.sd; v.td = op.td;
+  v.nsrc = int(vsrc_node_h_.size()); v.ntrg = int(vtrg_node_h_.size()); v.ngrp = int(vtgt_h_.size() / 8);
+  v.src_node = vsrc_d_.as<int>(); v.trg_node = vtrg_d_.as<int>();
+  v.depth = depth_d_.as<int>(); v.root_level = t.root_level;
+  v.deg = deg_d_.as<double>(); v.shell = shell_d_.as<int>(); v.twiddle = twiddle_d_.as<double2>();
+  v.UE = ue_d_.as<double>(); v.sz_ue = op.dat(UE);
+  v.DC = dc_d_.as<double>(); v.sz_dc = op.dat(DC);
+  v.halo = vhalo_d_.as<int>(); v.mask = vmask_d_.as<uint64_t>(); v.tgt = vtgt_d_.as<int>();
+  v.m2l = m2l_d_.as<double2>(); v.dspec = dspec_d_.as<double2>(); v.espec = espec_d_.as<double2>();
+  if ((rc = check(launch_vlist(v, s), "V list"))) return rc;
+
+  // X list S2L (:1233-1265)
+  mark(PH_XLIST);
+  sa.tasks = s2l_tasks_d_.as<SurfTask>(); sa.ntasks = int(s2l_tasks_h_.size());
+  sa.sam = sam_d_[DC].as<double>(); sa.n_sam = op.n_sam[DC];
+  sa.out = dc_d_.as<double>(); sa.sz_out = op.dat(DC);
+  if ((rc = check(launch_surface(kt_, sa, pu, s), "S2L"))) return rc;
+
+  // down pass (:1267-1382)
+  mark(PH_DOWN);
+  if ((rc = run_steps(dn_steps_))) return rc;
+
+  // U, W, X-fallback and L2T at the targets (:653-794, :1062-1231, :1383-1435)
+  mark(PH_NEAR);
+  NearArgs na{};
+  na.chunks = near_chunks_d_.as<NearChunk>(); na.nchunks = int(near_chunks_h_.size());
+  na.items = near_items_d_.as<int>();
+  na.spos = spos_d_.as<double>(); na.snor = snor_d_.as<double>(); na.sden = sden_d_.as<double>();
+  na.src_beg = sbeg_d_.as<int>(); na.src_num = snum_d_.as<int>();
+  na.geom = geom_d_.as<double4>();
+  na.sam_ue = sam_d_[UE].as<double>(); na.n_ue = op.n_sam[UE];
+  na.sam_de = sam_d_[DE].as<double>(); na.n_de = op.n_sam[DE];
+  na.UE = ue_d_.as<double>(); na.sz_ue = op.dat(UE);
+  na.DE = de_d_.as<double>(); na.sz_de = op.dat(DE);
+  na.tpos = tpos_d_.as<double>(); na.tval = tval_d_.as<double>();
+  if ((rc = check(launch_near(kt_, na, pu, pm, s), "near field"))) return rc;
+
+  // write back (:1490-1510)
+  mark(PH_FIN);
+  if ((rc = check(launch_scatter(tval_d_.as<double>(), tperm_d_.as<int>(), d_trg_val, nt_, TD, s), "scatter"))) return rc;
+  mark(PH_COUNT);
+  if (timing_) {
+    if ((rc = check(hipEventSynchronize(events_[PH_COUNT]), "sync"))) return rc;
+    for (int p = 0; p < PH_COUNT; p++) {
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, events_[p], events_[p + 1]);
+      phase_ms_[p] = ms;
+    }
+  }
+  return 0;
+}
+
+int FMM3d::direct(const double* d_src_den, int nchk, const double* d_chk_pos, double* d_chk_val,
+                  hipStream_t s) {
+  if (!setup_done_) return fail(-3, "direct called before setup");
+  int rc;
+  if (!direct_pos_d_.bytes()) {
+    if ((rc = check(direct_pos_d_.upload(spos_.data(), sizeof(double) * spos_.size()), "upload"))) return rc;
+    if ((rc = check(direct_nor_d_.upload(snor_.data(), sizeof(double) * snor_.size()), "upload"))) return rc;
+  }
+  const KParams<double> pu = make_params(kt_, coefs_.data());
+  return check(launch_direct(kt_, pu, ns_, direct_pos_d_.as<double>(), direct_nor_d_.as<double>(), d_src_den,
+                             nchk, d_chk_pos, d_chk_val, s), "direct sum");
+}
+
+}  // namespace kifmm

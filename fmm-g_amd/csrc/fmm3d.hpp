@@ -1,0 +1,143 @@
+// FMM3d: the MI355X counterpart of the reference's FMM3d_MPI (fmm3d_mpi.hpp:36-249)
+// on one rank.  Same configuration accessors (knl, np, ctr, rootLevel, ptsMax,
+// maxLevel), same setup()/evaluate() call order and error convention (int
+// return, 0 = success).  setup() builds the octree, lists and operator tables
+// on the host and leaves everything device-resident; evaluate() runs the whole
+// pass on one HIP stream.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "device.hpp"
+#include "let3d.hpp"
+#include "matmgnt3d.hpp"
+
+namespace kifmm {
+
+enum Precision { PREC_FP64 = 0, PREC_MIXED = 1 };
+
+// Phase names follow the reference's PETSc events (manage_petsc_events.cpp:77-143).
+enum Phase { PH_INI = 0, PH_UPW, PH_VLIST, PH_XLIST, PH_DOWN, PH_NEAR, PH_FIN, PH_COUNT };
+extern const char* const kPhaseNames[PH_COUNT];
+
+class DeviceBuffer {
+ public:
+  DeviceBuffer() = default;
+  ~DeviceBuffer();
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  hipError_t alloc(size_t bytes);
+  hipError_t upload(const void* host, size_t bytes);
+  void release();
+  template <typename T> T* as() const { return static_cast<T*>(ptr_); }
+  size_t bytes() const { return bytes_; }
+ private:
+  void* ptr_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+class FMM3d {
+ public:
+  FMM3d() = default;
+  ~FMM3d();
+
+  // --- configuration (fmm3d_mpi.hpp:170-193, knlmat3d_mpi.hpp:50-69) ---
+  int& kernelType() { return kt_; }
+  std::vector<double>& coefs() { return coefs_; }
+  int& np() { return np_; }
+  int& ptsMax() { return pts_max_; }
+  int& maxLevel() { return max_level_; }
+  double* ctr() { return ctr_; }
+  int& rootLevel() { return root_level_; }
+  int& precision() { return precision_; }
+  int srcDOF() const;
+  int trgDOF() const;
+
+  // Points are copied (the caller keeps ownership); normals may be null for
+  // kernels that do not use them.
+  int setPoints(int64_t ns, const double* spos, const double* snor, int64_t nt, const double* tpos);
+
+  // Host half of setup(): octree, lists, operator tables and task tables, no
+  // device calls (usable on a machine without a GPU).
+  int setupHost();
+  int setup();
+  // Device pointers, caller's point order (srcDOF / trgDOF interleaved).
+  int evaluate(const double* d_src_den, double* d_trg_val, hipStream_t stream);
+  // FP64 direct sum at the given targets (device pointers, caller's order).
+  int direct(const double* d_src_den, int nchk, const double* d_chk_pos, double* d_chk_val,
+             hipStream_t stream);
+
+  const Tree& tree() const { return tree_; }
+  const Operators* ops() const { return ops_.get(); }
+  const std::string& lastError() const { return err_; }
+  void setError(const std::string& e) { err_ = e; }
+  bool isSetup() const { return setup_done_; }
+  bool isHostReady() const { return host_ready_; }
+  int64_t numSrc() const { return ns_; }
+  int64_t numTrg() const { return nt_; }
+  const std::vector<double>& targetPositions() const { return tpos_; }
+  void enableTiming(bool on) { timing_ = on; }
+  std::array<float, PH_COUNT> phaseMs() const { return phase_ms_; }
+  // Work counts for the roofline (pairs per phase), filled at setup.
+  struct Work {
+    double near_pairs = 0, near_exact_pairs = 0, s2m_pairs = 0, s2l_pairs = 0;
+    double vlist_pairs = 0, n_vsrc = 0, n_vtrg = 0, n_vgrp = 0;
+    double up_cols = 0, m2m_cols = 0, dn_cols = 0, l2l_cols = 0;
+  };
+  const Work& work() const { return work_; }
+
+ private:
+  int fail(int code, const std::string& msg) { err_ = msg; return code; }
+  int check(hipError_t e, const char* what);
+  int build_tasks();
+  int upload_state();
+
+  int kt_ = 111, np_ = 6, pts_max_ = 64, max_level_ = 30, root_level_ = 1, precision_ = PREC_FP64;
+  std::vector<double> coefs_{1.0, 0.25};
+  double ctr_[3] = {0.5, 0.5, 0.5};
+  int64_t ns_ = 0, nt_ = 0;
+  std::vector<double> spos_, snor_, tpos_;
+  bool setup_done_ = false, host_ready_ = false, timing_ = false;
+  std::string err_;
+  Tree tree_;
+  std::shared_ptr<const Operators> ops_;
+  Work work_;
+  std::array<float, PH_COUNT> phase_ms_{};
+
+  // host task tables
+  // One level-batched GEMM launch of the up or down pass.  mat: 0 = M2M
+  // (the 8 child matrices side by side), 1 = UC->UE pinv, 2 = DC->DE pinv,
+  // 3+c = L2L for child c.  src/dst: 0 = UE, 1 = UC, 2 = DE, 3 = DC.
+  struct GemmStep {
+    int mat, src, dst, ncol, accumulate;
+    int64_t idx_off;  // bidx at idx_off, cidx at idx_off + ncol
+    int ks_off, rs_off;
+  };
+  std::vector<GemmStep> up_steps_, dn_steps_;
+  std::vector<int> lvl_idx_h_;
+  std::vector<double> scales_h_;
+  std::vector<NearChunk> near_chunks_h_;
+  std::vector<int> near_items_h_;
+  std::vector<SurfTask> s2m_tasks_h_, s2l_tasks_h_;
+  std::vector<int> surf_items_h_;
+  std::vector<int> vsrc_node_h_, vtrg_node_h_, vhalo_h_, vtgt_h_;
+  std::vector<uint64_t> vmask_h_;
+
+  // device state
+  DeviceBuffer spos_d_, snor_d_, tpos_d_, sperm_d_, tperm_d_, sden_d_, tval_d_;
+  DeviceBuffer geom_d_, depth_d_, sbeg_d_, snum_d_;
+  DeviceBuffer ue_d_, uc_d_, dc_d_, de_d_;
+  DeviceBuffer near_chunks_d_, near_items_d_, s2m_tasks_d_, s2l_tasks_d_, surf_items_d_;
+  DeviceBuffer lvl_idx_d_;
+  DeviceBuffer sam_d_[4], shell_d_, twiddle_d_, deg_d_, ones_d_, scales_d_;
+  DeviceBuffer pup_d_, pdn_d_, m2m_d_, l2l_d_, m2l_d_;
+  DeviceBuffer vsrc_d_, vtrg_d_, vhalo_d_, vtgt_d_, vmask_d_, dspec_d_, espec_d_;
+  DeviceBuffer direct_pos_d_, direct_nor_d_;
+  std::vector<hipEvent_t> events_;
+};
+
+}  // namespace kifmm

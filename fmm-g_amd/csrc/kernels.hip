@@ -1,0 +1,533 @@
+// HIP kernels of the FP64 evaluate pass for gfx950 (CDNA4, wave64).
+//
+// Phase -> kernel:
+//   load densities (fmm3d_eval_mpi.cpp:443-461)           k_gather
+//   S2M (:577) / X-list S2L (:1253)                       k_surface
+//   M2M, UC->UE, L2L, DC->DE (:583-592, :1371-1382)       k_gemm (level-batched, indexed columns)
+//   V-list (:1007-1059)                                   k_fft_fwd, k_hadamard, k_fft_inv
+//   U-list S2T, W-list S2T/M2T, X-list S2T, L2T
+//     (:773-785, :1146-1165, :1249, :1383-1435)            k_near (one pass per target chunk)
+//   write back (:1490-1510)                               k_scatter
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+
+namespace kifmm {
+
+// ---------------------------------------------------------------------------
+__global__ void k_gather(const double* __restrict__ in, const int* __restrict__ perm,
+                         double* __restrict__ out, int64_t n, int dof) {
+  int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int64_t o = perm[k];
+  for (int d = 0; d < dof; d++) out[k * dof + d] = in[o * dof + d];
+}
+
+__global__ void k_scatter(const double* __restrict__ in, const int* __restrict__ perm,
+                          double* __restrict__ out, int64_t n, int dof) {
+  int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int64_t o = perm[k];
+  for (int d = 0; d < dof; d++) out[o * dof + d] = in[k * dof + d];
+}
+
+__global__ void k_fill(double* p, int64_t n, double v) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  for (; i < n; i += int64_t(gridDim.x) * blockDim.x) p[i] = v;
+}
+
+hipError_t launch_gather(const double* in, const int* perm, double* out, int64_t n, int dof, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  k_gather<<<dim3(unsigned((n + 255) / 256)), dim3(256), 0, s>>>(in, perm, out, n, dof);
+  return hipGetLastError();
+}
+hipError_t launch_scatter(const double* in, const int* perm, double* out, int64_t n, int dof, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  k_scatter<<<dim3(unsigned((n + 255) / 256)), dim3(256), 0, s>>>(in, perm, out, n, dof);
+  return hipGetLastError();
+}
+hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  k_fill<<<dim3(unsigned(blocks)), dim3(256), 0, s>>>(p, n, v);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Near field: one workgroup per chunk of <= 128 targets of an evaluator leaf;
+// source items staged through LDS 128 at a time; one target per lane.
+// ---------------------------------------------------------------------------
+template <int KT>
+__global__ __launch_bounds__(kNearChunk) void k_near(NearArgs a, KParams<double> pu, KParams<double> pm) {
+  constexpr int KM = mm_kernel(KT);
+  constexpr int SD = src_dof(KT), TD = trg_dof(KT), sd = src_dof(KM);
+  constexpr int MAXD = SD > sd ? SD : sd;
+  static_assert(trg_dof(KM) == TD, "user and equivalent kernels share the target DOF");
+  __shared__ double sp[kNearChunk * 3];
+  __shared__ double sn[kNearChunk * 3];
+  __shared__ double sv[kNearChunk * MAXD];
+  const NearChunk ch = a.chunks[blockIdx.x];
+  const int lane = threadIdx.x;
+  const bool active = lane < ch.tnum;
+  double tx = 0, ty = 0, tz = 0;
+  if (active) {
+    const double* p = a.tpos + 3 * int64_t(ch.tbeg + lane);
+    tx = p[0]; ty = p[1]; tz = p[2];
+  }
+  double acc[TD];
+#pragma unroll
+  for (int t = 0; t < TD; t++) acc[t] = 0.0;
+
+  for (int it = ch.ibeg; it < ch.iend; it++) {
+    const int code = a.items[it];
+    const int node = code >> 2, kind = code & 3;
+    if (kind == ITEM_EXACT) {
+      const int b = a.src_beg[node], ns = a.src_num[node];
+      for (int base = 0; base < ns; base += kNearChunk) {
+        const int m = min(kNearChunk, ns - base);
+        __syncthreads();
+        if (lane < m) {
+          const int64_t g = int64_t(b + base + lane);
+          for (int d = 0; d < 3; d++) { sp[3 * lane + d] = a.spos[3 * g + d]; sn[3 * lane + d] = a.snor[3 * g + d]; }
+          for (int s = 0; s < SD; s++) sv[lane * SD + s] = a.sden[g * SD + s];
+        }
+        __syncthreads();
+        if (active) {
+          for (int j = 0; j < m; j++) {
+            double blk[SD * TD];
+            kblock<KT, double>(pu, tx - sp[3 * j], ty - sp[3 * j + 1], tz - sp[3 * j + 2], sn[3 * j],
+                               sn[3 * j + 1], sn[3 * j + 2], blk);
+#pragma unroll
+            for (int t = 0; t < TD; t++)
+#pragma unroll
+              for (int s = 0; s < SD; s++) acc[t] += blk[t * SD + s] * sv[j * SD + s];
+          }
+        }
+      }
+    } else {
+      const double4 g = a.geom[node];
+      const bool ue = kind == ITEM_UE;
+      const double* sam = ue ? a.sam_ue : a.sam_de;
+      const int ns = ue ? a.n_ue : a.n_de;
+      const double* den = ue ? a.UE + int64_t(node) * a.sz_ue : a.DE + int64_t(node) * a.sz_de;
+      for (int base = 0; base < ns; base += kNearChunk) {
+        const int m = min(kNearChunk, ns - base);
+        __syncthreads();
+        if (lane < m) {
+          const int q = base + lane;
+          sp[3 * lane] = g.x + g.w * sam[3 * q];
+          sp[3 * lane + 1] = g.y + g.w * sam[3 * q + 1];
+          sp[3 * lane + 2] = g.z + g.w * sam[3 * q + 2];
+          for (int s = 0; s < sd; s++) sv[lane * sd + s] = den[q * sd + s];
+        }
+        __syncthreads();
+        if (active) {
+          for (int j = 0; j < m; j++) {
+            double blk[sd * TD];
+            const double x = sp[3 * j], y = sp[3 * j + 1], z = sp[3 * j + 2];
+            kblock<KM, double>(pm, tx - x, ty - y, tz - z, x, y, z, blk);
+#pragma unroll
+            for (int t = 0; t < TD; t++)
+#pragma unroll
+              for (int s = 0; s < sd; s++) acc[t] += blk[t * sd + s] * sv[j * sd + s];
+          }
+        }
+      }
+    }
+  }
+  if (active) {
+    double* o = a.tval + int64_t(ch.tbeg + lane) * TD;
+#pragma unroll
+    for (int t = 0; t < TD; t++) o[t] = acc[t];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Exact sources -> check surface of a node (S2M into UC, S2L into DC).
+// One workgroup per task; each lane owns up to two surface points.
+// ---------------------------------------------------------------------------
+constexpr int kSurfThreads = 256;
+
+template <int KT>
+__global__ __launch_bounds__(kSurfThreads) void k_surface(SurfArgs a, KParams<double> pu) {
+  constexpr int SD = src_dof(KT), TD = trg_dof(KT);
+  __shared__ double sp[kSurfThreads * 3];
+  __shared__ double sn[kSurfThreads * 3];
+  __shared__ double sv[kSurfThreads * SD];
+  const SurfTask tk = a.tasks[blockIdx.x];
+  const double4 g = a.geom[tk.node];
+  const int lane = threadIdx.x;
+  double tp[2][3];
+  double acc[2][TD];
+  bool act[2];
+  for (int u = 0; u < 2; u++) {
+    const int q = lane + u * kSurfThreads;
+    act[u] = q < a.n_sam;
+    const int qq = act[u] ? q : 0;
+    tp[u][0] = g.x + g.w * a.sam[3 * qq];
+    tp[u][1] = g.y + g.w * a.sam[3 * qq + 1];
+    tp[u][2] = g.z + g.w * a.sam[3 * qq + 2];
+    for (int t = 0; t < TD; t++) acc[u][t] = 0.0;
+  }
+  for (int it = tk.ibeg; it < tk.iend; it++) {
+    const int node = a.items[it];
+    const int b = a.src_beg[node], ns = a.src_num[node];
+    for (int base = 0; base < ns; base += kSurfThreads) {
+      const int m = min(kSurfThreads, ns - base);
+      __syncthreads();
+      if (lane < m) {
+        const int64_t gg = int64_t(b + base + lane);
+        for (int d = 0; d < 3; d++) { sp[3 * lane + d] = a.spos[3 * gg + d]; sn[3 * lane + d] = a.snor[3 * gg + d]; }
+        for (int s = 0; s < SD; s++) sv[lane * SD + s] = a.sden[gg * SD + s];
+      }
+      __syncthreads();
+      for (int j = 0; j < m; j++) {
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+          if (!act[u]) continue;
+          double blk[SD * TD];
+          kblock<KT, double>(pu, tp[u][0] - sp[3 * j], tp[u][1] - sp[3 * j + 1], tp[u][2] - sp[3 * j + 2],
+                             sn[3 * j], sn[3 * j + 1], sn[3 * j + 2], blk);
+#pragma unroll
+          for (int t = 0; t < TD; t++)
+#pragma unroll
+            for (int s = 0; s < SD; s++) acc[u][t] += blk[t * SD + s] * sv[j * SD + s];
+        }
+      }
+    }
+  }
+  double* o = a.out + int64_t(tk.node) * a.sz_out;
+  for (int u = 0; u < 2; u++) {
+    if (!act[u]) continue;
+    const int q = lane + u * kSurfThreads;
+    for (int t = 0; t < TD; t++) o[q * TD + t] += acc[u][t];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Indexed batched GEMM: C[:, cidx[j]] = (acc ? C : 0) + rs (.) (A * (ks (.) B[:, bidx[j]]))
+// A is M x K column-major; columns of B/C are contiguous.  64x64 tile per
+// workgroup, 4x4 outputs per lane, K staged through LDS in slabs of 16.
+// ---------------------------------------------------------------------------
+constexpr int GT_M = 64, GT_N = 64, GT_K = 16;
+
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
+  __shared__ double As[GT_K][GT_M + 1];
+  __shared__ double Bs[GT_K][GT_N + 1];
+  __shared__ int64_t bcol[GT_N];
+  const int m0 = blockIdx.x * GT_M, n0 = blockIdx.y * GT_N;
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  if (tid < GT_N) {
+    const int j = n0 + tid;
+    bcol[tid] = j < g.ncol ? int64_t(g.bidx[j]) * g.ldb : int64_t(-1);
+  }
+  double c[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) c[i][j] = 0.0;
+  __syncthreads();
+  for (int k0 = 0; k0 < g.K; k0 += GT_K) {
+    for (int e = tid; e < GT_M * GT_K; e += 256) {
+      const int mm = e % GT_M, kk = e / GT_M;
+      const int m = m0 + mm, k = k0 + kk;
+      As[kk][mm] = (m < g.M && k < g.K) ? g.A[int64_t(k) * g.M + m] : 0.0;
+    }
+    for (int e = tid; e < GT_N * GT_K; e += 256) {
+      const int kk = e % GT_K, nn = e / GT_K;
+      const int k = k0 + kk;
+      const int64_t col = bcol[nn];
+      Bs[kk][nn] = (col >= 0 && k < g.K) ? g.B[col + k] * g.kscale[k % g.ksd] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GT_K; kk++) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) av[i] = As[kk][ty + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; j++) bv[j] = Bs[kk][tx + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) c[i][j] += av[i] * bv[j];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int n = n0 + tx + 16 * j;
+    if (n >= g.ncol) continue;
+    double* col = g.C + int64_t(g.cidx[n]) * g.ldc;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int m = m0 + ty + 16 * i;
+      if (m >= g.M) continue;
+      const double v = g.rscale[m % g.rsd] * c[i][j];
+      col[m] = g.accumulate ? col[m] + v : v;
+    }
+  }
+}
+
+hipError_t launch_gemm(const GemmArgs& g, hipStream_t s) {
+  if (g.ncol <= 0) return hipSuccess;
+  dim3 grid(unsigned((g.M + GT_M - 1) / GT_M), unsigned((g.ncol + GT_N - 1) / GT_N));
+  k_gemm<<<grid, dim3(256), 0, s>>>(g);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// V list.  Forward: UE samples (scaled 2^(l*deg)) scattered onto the (2np)^3
+// grid and transformed (r2c, FFTW layout).  Only the np^3 block at offset np/2
+// is non-zero, so the three DFT passes run over that block.
+// ---------------------------------------------------------------------------
+constexpr int kFftThreads = 256;
+constexpr int kMaxNp = 8;
+constexpr int kMaxN = 2 * kMaxNp, kMaxH = kMaxN / 2 + 1;
+
+__device__ inline double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+
+__global__ __launch_bounds__(kFftThreads) void k_fft_fwd(VlistArgs v) {
+  __shared__ double xb[kMaxNp * kMaxNp * kMaxNp];
+  __shared__ double2 t1[kMaxNp * kMaxNp * kMaxH];
+  __shared__ double2 t2[kMaxNp * kMaxN * kMaxH];
+  __shared__ double2 w[kMaxN];
+  const int slot = blockIdx.x;
+  const int node = v.src_node[slot];
+  const int np = v.np, n = v.n, h = n / 2 + 1, o = np / 2, H = v.H;
+  const int tid = threadIdx.x;
+  const int nsam = np * np * np - (np - 2) * (np - 2) * (np - 2);
+  if (tid < n) w[tid] = v.twiddle[tid];
+  const int l = v.depth[node] + v.root_level;
+  for (int s = 0; s < v.sd; s++) {
+    const double scale = ldexp(1.0, int(l * v.deg[s]));
+    __syncthreads();
+    for (int e = tid; e < np * np * np; e += kFftThreads) xb[e] = 0.0;
+    __syncthreads();
+    for (int q = tid; q < nsam; q += kFftThreads)
+      xb[v.shell[q]] = v.UE[int64_t(node) * v.sz_ue + q * v.sd + s] * scale;
+    __syncthreads();
+    // along x (c): real -> half spectrum
+    for (int e = tid; e < np * np * h; e += kFftThreads) {
+      const int kc = e % h, ab = e / h;
+      double2 acc = make_double2(0, 0);
+      for (int c = 0; c < np; c++) {
+        const double2 tw = w[((c + o) * kc) % n];
+        const double xv = xb[ab * np + c];
+        acc.x += xv * tw.x; acc.y += xv * tw.y;
+      }
+      t1[e] = acc;
+    }
+    __syncthreads();
+    // along y (b)
+    for (int e = tid; e < np * n * h; e += kFftThreads) {
+      const int kc = e % h, kb = (e / h) % n, a = e / (h * n);
+      double2 acc = make_double2(0, 0);
+      for (int b = 0; b < np; b++) acc = cadd(acc, cmul(t1[(a * np + b) * h + kc], w[((b + o) * kb) % n]));
+      t2[e] = acc;
+    }
+    __syncthreads();
+    // along z (a) -> global, frequency index (ka*n+kb)*h+kc
+    double2* out = v.dspec + (int64_t(slot) * v.sd + s) * H;
+    for (int e = tid; e < H; e += kFftThreads) {
+      const int kc = e % h, kb = (e / h) % n, ka = e / (h * n);
+      double2 acc = make_double2(0, 0);
+      for (int a = 0; a < np; a++) acc = cadd(acc, cmul(t2[(a * n + kb) * h + kc], w[((a + o) * ka) % n]));
+      out[e] = acc;
+    }
+  }
+}
+
+// Hadamard accumulation: one workgroup per sibling group (the 8 children of a
+// parent) and 64 frequencies; lane = frequency.  Sources are the children of
+// the parent's colleagues ("halo", 6x6x6 slots); membership of each
+// (target child, halo slot) pair in the V list is a bit mask built from the
+// CSR lists, so the pairs are exactly the reference's.
+template <int SD, int TD>
+__global__ __launch_bounds__(64) void k_hadamard(VlistArgs v, double nrm) {
+  const int g = blockIdx.x;
+  const int f = blockIdx.y * 64 + threadIdx.x;
+  const int H = v.H;
+  const bool ok = f < H;
+  const int ff = ok ? f : 0;
+  double2 acc[8][TD];
+#pragma unroll
+  for (int t = 0; t < 8; t++)
+#pragma unroll
+    for (int i = 0; i < TD; i++) acc[t][i] = make_double2(0, 0);
+  const uint64_t* mk = v.mask + int64_t(g) * 32;
+  const int* halo = v.halo + int64_t(g) * 216;
+  for (int r = 0; r < 216; r++) {
+    const int s = halo[r];
+    if (s < 0) continue;
+    double2 d[SD];
+#pragma unroll
+    for (int j = 0; j < SD; j++) d[j] = v.dspec[(int64_t(s) * SD + j) * H + ff];
+    const int rx = r / 36, ry = (r / 6) % 6, rz = r % 6;  // halo coordinate + 2
+#pragma unroll
+    for (int tau = 0; tau < 8; tau++) {
+      if (!((mk[tau * 4 + (r >> 6)] >> (r & 63)) & 1ull)) continue;
+      const int ox = rx - 2 - (tau >> 2), oy = ry - 2 - ((tau >> 1) & 1), oz = rz - 2 - (tau & 1);
+      const int slot = (ox + 3) * 49 + (oy + 3) * 7 + (oz + 3);
+      const double2* T = v.m2l + int64_t(slot) * SD * TD * H;
+#pragma unroll
+      for (int i = 0; i < TD; i++)
+#pragma unroll
+        for (int j = 0; j < SD; j++) acc[tau][i] = cadd(acc[tau][i], cmul(T[int64_t(j * TD + i) * H + ff], d[j]));
+    }
+  }
+  if (!ok) return;
+#pragma unroll
+  for (int tau = 0; tau < 8; tau++) {
+    const int t = v.tgt[g * 8 + tau];
+    if (t < 0) continue;
+#pragma unroll
+    for (int i = 0; i < TD; i++)
+      v.espec[(int64_t(t) * TD + i) * H + f] = make_double2(acc[tau][i].x * nrm, acc[tau][i].y * nrm);
+  }
+}
+
+// Inverse: unnormalised c2r of each target spectrum, evaluated only on the
+// surface samples, added into DC (effVal2PlnVal + regVal2SamVal).
+__global__ __launch_bounds__(kFftThreads) void k_fft_inv(VlistArgs v) {
+  __shared__ double2 xs[kMaxN * kMaxN * kMaxH];
+  __shared__ double2 y1[kMaxNp * kMaxN * kMaxH];
+  __shared__ double2 y2[kMaxNp * kMaxNp * kMaxH];
+  __shared__ double2 w[kMaxN];
+  const int slot = blockIdx.x;
+  const int node = v.trg_node[slot];
+  const int np = v.np, n = v.n, h = n / 2 + 1, o = np / 2, H = v.H;
+  const int tid = threadIdx.x;
+  const int nsam = np * np * np - (np - 2) * (np - 2) * (np - 2);
+  if (tid < n) w[tid] = make_double2(v.twiddle[tid].x, -v.twiddle[tid].y);  // e^{+2 pi i m/n}
+  for (int i = 0; i < v.td; i++) {
+    __syncthreads();
+    const double2* in = v.espec + (int64_t(slot) * v.td + i) * H;
+    for (int e = tid; e < H; e += kFftThreads) xs[e] = in[e];
+    __syncthreads();
+    // along z (a), only block rows a' in [0,np)
+    for (int e = tid; e < np * n * h; e += kFftThreads) {
+      const int kc = e % h, kb = (e / h) % n, a = e / (h * n);
+      double2 acc = make_double2(0, 0);
+      for (int ka = 0; ka < n; ka++) acc = cadd(acc, cmul(xs[(ka * n + kb) * h + kc], w[((a + o) * ka) % n]));
+      y1[e] = acc;
+    }
+    __syncthreads();
+    // along y (b), only b' in [0,np)
+    for (int e = tid; e < np * np * h; e += kFftThreads) {
+      const int kc = e % h, b = (e / h) % np, a = e / (h * np);
+      double2 acc = make_double2(0, 0);
+      for (int kb = 0; kb < n; kb++) acc = cadd(acc, cmul(y1[(a * n + kb) * h + kc], w[((b + o) * kb) % n]));
+      y2[e] = acc;
+    }
+    __syncthreads();
+    // along x (c), c2r at the surface samples
+    double* dc = v.DC + int64_t(node) * v.sz_dc;
+    for (int q = tid; q < nsam; q += kFftThreads) {
+      const int blk = v.shell[q];
+      const int c = blk % np, ab = blk / np;
+      const double2* yy = y2 + ab * h;
+      const int cg = c + o;
+      double val = yy[0].x + yy[n / 2].x * ((cg & 1) ? -1.0 : 1.0);
+      for (int kc = 1; kc < n / 2; kc++) {
+        const double2 tw = w[(cg * kc) % n];
+        val += 2.0 * (yy[kc].x * tw.x - yy[kc].y * tw.y);
+      }
+      dc[q * v.td + i] += val;
+    }
+  }
+}
+
+hipError_t launch_vlist(const VlistArgs& v, hipStream_t s) {
+  if (v.ngrp <= 0) return hipSuccess;
+  if (v.np > kMaxNp) return hipErrorInvalidValue;
+  k_fft_fwd<<<dim3(unsigned(v.nsrc)), dim3(kFftThreads), 0, s>>>(v);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const double nrm = 1.0 / double(v.n * v.n * v.n);
+  dim3 grid(unsigned(v.ngrp), unsigned((v.H + 63) / 64));
+  if (v.sd == 1 && v.td == 1) k_hadamard<1, 1><<<grid, dim3(64), 0, s>>>(v, nrm);
+  else if (v.sd == 4 && v.td == 3) k_hadamard<4, 3><<<grid, dim3(64), 0, s>>>(v, nrm);
+  else if (v.sd == 3 && v.td == 3) k_hadamard<3, 3><<<grid, dim3(64), 0, s>>>(v, nrm);
+  else return hipErrorInvalidValue;
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  k_fft_inv<<<dim3(unsigned(v.ntrg)), dim3(kFftThreads), 0, s>>>(v);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// FP64 direct sum at sampled targets (dense3d_mpi.cpp:122-129): one workgroup
+// per target, lanes stride over the sources, LDS tree reduction.
+// ---------------------------------------------------------------------------
+template <int KT>
+__global__ __launch_bounds__(256) void k_direct(KParams<double> p, int64_t ns, const double* __restrict__ spos,
+                                                const double* __restrict__ snor, const double* __restrict__ sden,
+                                                const double* __restrict__ tpos, double* __restrict__ tval) {
+  constexpr int SD = src_dof(KT), TD = trg_dof(KT);
+  __shared__ double red[256 * TD];
+  const int i = blockIdx.x;
+  const double tx = tpos[3 * i], ty = tpos[3 * i + 1], tz = tpos[3 * i + 2];
+  double acc[TD];
+  for (int t = 0; t < TD; t++) acc[t] = 0.0;
+  for (int64_t j = threadIdx.x; j < ns; j += 256) {
+    double blk[SD * TD];
+    kblock<KT, double>(p, tx - spos[3 * j], ty - spos[3 * j + 1], tz - spos[3 * j + 2], snor[3 * j],
+                       snor[3 * j + 1], snor[3 * j + 2], blk);
+    for (int t = 0; t < TD; t++)
+      for (int s = 0; s < SD; s++) acc[t] += blk[t * SD + s] * sden[j * SD + s];
+  }
+  for (int t = 0; t < TD; t++) red[t * 256 + threadIdx.x] = acc[t];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (int(threadIdx.x) < w)
+      for (int t = 0; t < TD; t++) red[t * 256 + threadIdx.x] += red[t * 256 + threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < TD) tval[int64_t(i) * TD + threadIdx.x] = red[threadIdx.x * 256];
+}
+
+// ---------------------------------------------------------------------------
+#define KIFMM_USER_KERNELS(X) X(111) X(121) X(311) X(312) X(321) X(322) X(511) X(521)
+
+hipError_t launch_near(int kt, const NearArgs& a, const KParams<double>& pu, const KParams<double>& pm,
+                       hipStream_t s) {
+  if (a.nchunks <= 0) return hipSuccess;
+  switch (kt) {
+#define KIFMM_CASE(K) case K: k_near<K><<<dim3(unsigned(a.nchunks)), dim3(kNearChunk), 0, s>>>(a, pu, pm); break;
+    KIFMM_USER_KERNELS(KIFMM_CASE)
+#undef KIFMM_CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_surface(int kt, const SurfArgs& a, const KParams<double>& pu, hipStream_t s) {
+  if (a.ntasks <= 0) return hipSuccess;
+  if (a.n_sam > 2 * kSurfThreads) return hipErrorInvalidValue;
+  switch (kt) {
+#define KIFMM_CASE(K) case K: k_surface<K><<<dim3(unsigned(a.ntasks)), dim3(kSurfThreads), 0, s>>>(a, pu); break;
+    KIFMM_USER_KERNELS(KIFMM_CASE)
+#undef KIFMM_CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_direct(int kt, const KParams<double>& p, int64_t ns, const double* spos, const double* snor,
+                         const double* sden, int nt, const double* tpos, double* tval, hipStream_t s) {
+  if (nt <= 0) return hipSuccess;
+  switch (kt) {
+#define KIFMM_CASE(K) case K: k_direct<K><<<dim3(unsigned(nt)), dim3(256), 0, s>>>(p, ns, spos, snor, sden, tpos, tval); break;
+    KIFMM_USER_KERNELS(KIFMM_CASE)
+#undef KIFMM_CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace kifmm

@@ -1,0 +1,231 @@
+"""Python binding of libkifmm.so (the MI355X KIFMM3d evaluate path).
+
+A thin ctypes layer over the C ABI in ``include/kifmm.h``.  ``FMM3d`` mirrors the
+reference's ``FMM3d_MPI`` operator interface (fmm3d_mpi/fmm3d_mpi.hpp:36-249,
+knlmat3d_mpi.hpp:29-80): configure, ``set_points``, ``setup()``, then
+``evaluate()`` any number of times.  Every failure raises ``KifmmError`` with the
+library's message; there is no fallback path: if the native library is missing
+the import of a handle fails.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libkifmm.so")
+_LIB = None
+
+P_D = C.POINTER(C.c_double)
+P_F = C.POINTER(C.c_float)
+P_I = C.POINTER(C.c_int)
+P_L = C.POINTER(C.c_int64)
+H = C.c_void_p
+
+PREC = {"fp64": 0, "mixed": 1}
+PHASES = ["EvlIni", "EvlUpwComp", "EvlVList", "EvlXList", "EvlCombine", "EvlUList", "EvlFinalize"]
+WORK = ["near_pairs", "near_exact_pairs", "s2m_pairs", "s2l_pairs", "vlist_pairs", "n_vsrc",
+        "n_vtrg", "n_vgrp", "up_cols", "m2m_cols", "dn_cols", "l2l_cols"]
+
+# exported symbol -> (restype, argtypes)
+SIGNATURES = {
+    "kifmm_create": (C.c_int, [C.c_int, P_D, C.c_int, C.c_int, C.c_int, C.c_int, P_D, C.c_int,
+                               C.c_int, C.POINTER(H)]),
+    "kifmm_create_from_options": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(H)]),
+    "kifmm_destroy": (C.c_int, [H]),
+    "kifmm_set_points": (C.c_int, [H, C.c_int64, P_D, P_D, C.c_int64, P_D]),
+    "kifmm_setup": (C.c_int, [H]),
+    "kifmm_setup_host": (C.c_int, [H]),
+    "kifmm_src_dof": (C.c_int, [H]),
+    "kifmm_trg_dof": (C.c_int, [H]),
+    "kifmm_evaluate": (C.c_int, [H, P_D, P_D]),
+    "kifmm_evaluate_device": (C.c_int, [H, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kifmm_direct_device": (C.c_int, [H, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kifmm_check": (C.c_int, [H, P_D, P_D, C.c_int, C.c_uint64, P_D]),
+    "kifmm_tree_info": (C.c_int, [H, P_L]),
+    "kifmm_get_tree": (C.c_int, [H] + [P_I] * 11),
+    "kifmm_get_list": (C.c_int, [H, C.c_int, P_L, P_I]),
+    "kifmm_get_operator": (C.c_int, [H, C.c_int, P_D, P_L]),
+    "kifmm_set_timing": (C.c_int, [H, C.c_int]),
+    "kifmm_phase_times": (C.c_int, [H, P_F, C.c_int]),
+    "kifmm_phase_name": (C.c_char_p, [C.c_int]),
+    "kifmm_work_counts": (C.c_int, [H, P_D, C.c_int]),
+    "kifmm_options_lookup": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int]),
+    "kifmm_last_error": (C.c_char_p, [H]),
+}
+
+
+class KifmmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("kifmm error %d: %s" % (code, msg))
+        self.code = code
+
+
+def lib():
+    """Load libkifmm.so (built in-tree by ``make -C fmm-g_amd``)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise KifmmError(-1, "native library not built: %s" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _f64(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if n is not None and a.size != n:
+        raise ValueError("expected %d values, got %d" % (n, a.size))
+    return a
+
+
+def _pd(a):
+    return a.ctypes.data_as(P_D) if a is not None else None
+
+
+def options_lookup(path, name):
+    """Value of option `name` in a PETSc-style options file ('' for a flag, None if absent)."""
+    buf = C.create_string_buffer(1024)
+    rc = lib().kifmm_options_lookup(path.encode(), name.encode(), buf, 1024)
+    if rc < 0:
+        raise KifmmError(rc, lib().kifmm_last_error(None).decode())
+    return buf.value.decode() if rc == 1 else None
+
+
+class FMM3d:
+    """KIFMM3d evaluate operator on one GPU (reference: FMM3d_MPI)."""
+
+    def __init__(self, kernel_type=111, coefs=(1.0, 0.25), np=6, pts_max=64, max_level=30,
+                 ctr=(0.5, 0.5, 0.5), root_level=1, precision="fp64", _handle=None):
+        L = lib()
+        self._h = H()
+        if _handle is not None:
+            self._h = _handle
+        else:
+            c = _f64(coefs)
+            cc = _f64(ctr, 3)
+            rc = L.kifmm_create(int(kernel_type), _pd(c), c.size, int(np), int(pts_max),
+                                int(max_level), _pd(cc), int(root_level), PREC[precision],
+                                C.byref(self._h))
+            if rc != 0:
+                raise KifmmError(rc, L.kifmm_last_error(None).decode())
+        self.kernel_type = kernel_type
+        self.n_src = self.n_trg = 0
+
+    @classmethod
+    def from_options(cls, path, prefix=None):
+        L = lib()
+        h = H()
+        rc = L.kifmm_create_from_options(path.encode(), prefix.encode() if prefix else None,
+                                         C.byref(h))
+        if rc != 0:
+            raise KifmmError(rc, L.kifmm_last_error(None).decode())
+        return cls(_handle=h)
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise KifmmError(rc, lib().kifmm_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            lib().kifmm_destroy(self._h)
+            self._h = H()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def src_dof(self):
+        return lib().kifmm_src_dof(self._h)
+
+    @property
+    def trg_dof(self):
+        return lib().kifmm_trg_dof(self._h)
+
+    def set_points(self, src_pos, src_nor=None, trg_pos=None):
+        sp = _f64(src_pos).reshape(-1)
+        tp = sp if trg_pos is None else _f64(trg_pos).reshape(-1)
+        sn = None if src_nor is None else _f64(src_nor, sp.size)
+        self.n_src, self.n_trg = sp.size // 3, tp.size // 3
+        self._chk(lib().kifmm_set_points(self._h, self.n_src, _pd(sp), _pd(sn), self.n_trg, _pd(tp)))
+
+    def setup(self):
+        self._chk(lib().kifmm_setup(self._h))
+
+    def setup_host(self):
+        self._chk(lib().kifmm_setup_host(self._h))
+
+    def evaluate(self, src_den):
+        """Host arrays in, host array out (FMM3d_MPI::evaluate)."""
+        d = _f64(src_den, self.n_src * self.src_dof)
+        out = np.zeros(self.n_trg * self.trg_dof)
+        self._chk(lib().kifmm_evaluate(self._h, _pd(d), _pd(out)))
+        return out
+
+    def evaluate_device(self, d_src_den, d_trg_val, stream=None):
+        """Device-resident float64 tensors (e.g. torch CUDA/HIP tensors)."""
+        self._chk(lib().kifmm_evaluate_device(self._h, C.c_void_p(d_src_den.data_ptr()),
+                                              C.c_void_p(d_trg_val.data_ptr()),
+                                              C.c_void_p(stream) if stream else None))
+
+    def direct_device(self, d_src_den, d_chk_pos, d_chk_val, stream=None):
+        n = d_chk_pos.numel() // 3
+        self._chk(lib().kifmm_direct_device(self._h, C.c_void_p(d_src_den.data_ptr()), n,
+                                            C.c_void_p(d_chk_pos.data_ptr()),
+                                            C.c_void_p(d_chk_val.data_ptr()),
+                                            C.c_void_p(stream) if stream else None))
+
+    def check(self, src_den, trg_val, numchk, seed=0):
+        d = _f64(src_den)
+        v = _f64(trg_val)
+        r = C.c_double()
+        self._chk(lib().kifmm_check(self._h, _pd(d), _pd(v), int(numchk), int(seed), C.byref(r)))
+        return r.value
+
+    def tree(self):
+        info = np.zeros(8, np.int64)
+        self._chk(lib().kifmm_tree_info(self._h, info.ctypes.data_as(P_L)))
+        n, ns, nt = int(info[0]), int(info[6]), int(info[7])
+        names = ["parent", "child", "depth", "path", "tag", "src_beg", "src_num", "trg_beg",
+                 "trg_num", "src_perm", "trg_perm"]
+        sizes = [n, n, n, 3 * n, n, n, n, n, n, ns, nt]
+        arrs = [np.zeros(max(s, 1), np.int32) for s in sizes]
+        self._chk(lib().kifmm_get_tree(self._h, *[a.ctypes.data_as(P_I) for a in arrs]))
+        out = {k: a[:s] for k, a, s in zip(names, arrs, sizes)}
+        out["path"] = out["path"].reshape(n, 3)
+        out["max_depth"] = int(info[1])
+        out["lists"] = {}
+        for kind, name in enumerate("uvwx"):
+            off = np.zeros(n + 1, np.int64)
+            lst = np.zeros(max(int(info[2 + kind]), 1), np.int32)
+            self._chk(lib().kifmm_get_list(self._h, kind, off.ctypes.data_as(P_L),
+                                           lst.ctypes.data_as(P_I)))
+            out["lists"][name] = (off, lst[:int(info[2 + kind])])
+        return out
+
+    def operator(self, which):
+        cnt = C.c_int64()
+        self._chk(lib().kifmm_get_operator(self._h, which, None, C.byref(cnt)))
+        out = np.zeros(cnt.value)
+        self._chk(lib().kifmm_get_operator(self._h, which, _pd(out), C.byref(cnt)))
+        return out
+
+    def set_timing(self, on=True):
+        self._chk(lib().kifmm_set_timing(self._h, 1 if on else 0))
+
+    def phase_times(self):
+        ms = (C.c_float * len(PHASES))()
+        self._chk(lib().kifmm_phase_times(self._h, ms, len(PHASES)))
+        return dict(zip(PHASES, list(ms)))
+
+    def work_counts(self):
+        w = np.zeros(len(WORK))
+        self._chk(lib().kifmm_work_counts(self._h, _pd(w), len(WORK)))
+        return dict(zip(WORK, w.tolist()))

@@ -1,0 +1,129 @@
+/*
+ * kifmm.h -- C ABI of the MI355X KIFMM3d evaluate path (libkifmm.so).
+ *
+ * Drop-in for the evaluate path of the reference's FMM3d_MPI on one MPI rank.
+ * Every entry point below replaces the reference interface cited beside it
+ * (paths relative to the reference root).  Plain pointers and sizes only; every
+ * call returns an int status (0 = success, negative = error class) and never
+ * aborts; kifmm_last_error() gives the message.  Handles are independent and
+ * thread-compatible (one host thread per handle at a time).
+ *
+ * Layouts (the reference's, tt0.cpp:472-476, fmm3d_eval_mpi.cpp:455,1503):
+ *   positions / normals: xyz interleaved, 3 doubles per point;
+ *   densities: srcDOF doubles per point; values: trgDOF doubles per point.
+ */
+#ifndef KIFMM_H
+#define KIFMM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KIFMM_OK 0
+#define KIFMM_ERR_ARG (-2)      /* bad argument or unsupported configuration */
+#define KIFMM_ERR_STATE (-3)    /* call out of order (e.g. evaluate before setup) */
+#define KIFMM_ERR_OPTIONS (-4)  /* options file missing or a required option absent */
+#define KIFMM_ERR_HIP (-5)      /* HIP runtime error (no device, allocation, launch) */
+#define KIFMM_ERR_INTERNAL (-6)
+
+#define KIFMM_PREC_FP64 0
+#define KIFMM_PREC_MIXED 1
+
+typedef struct kifmm_fmm *kifmm_handle;
+
+/* Constructor + configuration: FMM3d_MPI(prefix) and its setters knl(), np(),
+ * ctr(), rootLevel() (fmm3d_mpi.hpp:170-193, tt0.cpp:429-437) plus the
+ * -fmm3d_let3d_ptsMax / -maxLevel options (let3d_mpi.cpp:231-232). */
+int kifmm_create(int kernel_type, const double *coefs, int ncoefs, int np, int pts_max,
+                 int max_level, const double ctr[3], int root_level, int precision,
+                 kifmm_handle *out);
+
+/* Same, configured from a PETSc-style options file as tt0 reads it
+ * (tt0.cpp:150-183, fmm3d_setup_mpi.cpp:48, let3d_mpi.cpp:231-232):
+ * -kt, -<prefix>np, -<prefix>let3d_ptsMax, -<prefix>let3d_maxLevel; optional
+ * -kifmm_mu, -kifmm_nu (default 1, 0.25 as tt0.cpp:179-183) and
+ * -kifmm_precision fp64|mixed.  prefix NULL means "fmm3d_". */
+int kifmm_create_from_options(const char *options_path, const char *prefix, kifmm_handle *out);
+
+/* ~FMM3d_MPI (fmm3d_mpi.cpp:41-76). */
+int kifmm_destroy(kifmm_handle h);
+
+/* srcPos() / srcNor() / trgPos() setters (knlmat3d_mpi.hpp:50-56).  Copies the
+ * arrays; src_nor may be NULL for kernels without normals. */
+int kifmm_set_points(kifmm_handle h, int64_t n_src, const double *src_pos, const double *src_nor,
+                     int64_t n_trg, const double *trg_pos);
+
+/* KnlMat3d_MPI::setup() (knlmat3d_mpi.hpp:59; fmm3d_setup_mpi.cpp:41). */
+int kifmm_setup(kifmm_handle h);
+
+/* Host half of setup: octree, lists, operator tables and task tables, without
+ * touching the GPU (the tree, list, operator and work-count exports below need
+ * only this).  kifmm_setup() performs it too.  No reference counterpart: the
+ * reference's setup has no device half. */
+int kifmm_setup_host(kifmm_handle h);
+
+/* srcDOF() / trgDOF() (knlmat3d_mpi.hpp:67-69); negative on error. */
+int kifmm_src_dof(kifmm_handle h);
+int kifmm_trg_dof(kifmm_handle h);
+
+/* KnlMat3d_MPI::evaluate(Vec srcDen, Vec trgVal) (knlmat3d_mpi.hpp:62;
+ * fmm3d_eval_mpi.cpp:396).  Host pointers; copies in, evaluates on the GPU,
+ * copies out, synchronous.  trg_val is overwritten. */
+int kifmm_evaluate(kifmm_handle h, const double *src_den, double *trg_val);
+
+/* Same with device pointers, enqueued on `stream` (a hipStream_t; NULL = the
+ * default stream); returns once enqueued. */
+int kifmm_evaluate_device(kifmm_handle h, const double *d_src_den, double *d_trg_val, void *stream);
+
+/* FP64 direct sum at nchk device-resident targets with the user kernel
+ * (Dense3d_MPI::evaluate, dense3d_mpi.cpp:75-141). */
+int kifmm_direct_device(kifmm_handle h, const double *d_src_den, int nchk, const double *d_chk_pos,
+                        double *d_chk_val, void *stream);
+
+/* FMM3d_MPI::check (fmm3d_check_mpi.cpp:34-173): relative 2-norm error of
+ * trg_val against the direct sum at numchk targets sampled with `seed`
+ * (all targets when numchk >= n_trg).  Host pointers. */
+int kifmm_check(kifmm_handle h, const double *src_den, const double *trg_val, int numchk,
+                uint64_t seed, double *rerr);
+
+/* Tree / list export for parity tests (Let3d_MPI accessors, let3d_mpi.hpp).
+ * info: n_nodes, max_depth, |U|, |V|, |W|, |X| (total entries), n_src, n_trg. */
+int kifmm_tree_info(kifmm_handle h, int64_t info[8]);
+int kifmm_get_tree(kifmm_handle h, int *parent, int *child, int *depth, int *path, int *tag,
+                   int *src_beg, int *src_num, int *trg_beg, int *trg_num, int *src_perm,
+                   int *trg_perm);
+/* kind: 0 U, 1 V, 2 W, 3 X (Node::Unodes()..Xnodes()); off has n_nodes+1 entries. */
+int kifmm_get_list(kifmm_handle h, int kind, int64_t *off, int *lst);
+
+/* Operator tables of MatMgnt3d_MPI (matmgnt3d_mpi.cpp:107-431), column-major
+ * FP64: which 0 = UC->UE pinv, 1 = DC->DE pinv, 2 = 8 M2M, 3 = 8 L2L,
+ * 4 = M2L spectra [343][sd*td][H] complex.  With out == NULL only *count
+ * (number of doubles) is returned. */
+int kifmm_get_operator(kifmm_handle h, int which, double *out, int64_t *count);
+
+/* Per-phase device time of the last evaluate (PETSc event names of
+ * manage_petsc_events.cpp:77-143) when timing is on; ms has KIFMM_NPHASE slots. */
+#define KIFMM_NPHASE 7
+int kifmm_set_timing(kifmm_handle h, int on);
+int kifmm_phase_times(kifmm_handle h, float *ms, int n);
+const char *kifmm_phase_name(int phase);
+/* Work counts from the built tree, used for roofline figures: near pairs, exact near
+ * pairs, S2M pairs, S2L pairs, V pairs, #V sources, #V targets, #V groups,
+ * UC->UE columns, M2M columns, DC->DE columns, L2L columns. */
+#define KIFMM_NWORK 12
+int kifmm_work_counts(kifmm_handle h, double *out, int n);
+
+/* PetscOptionsGet* over an options file (%/# comments, case-insensitive
+ * names): 1 if found (value copied, "" for a flag), 0 if absent, <0 on error. */
+int kifmm_options_lookup(const char *options_path, const char *name, char *value, int len);
+
+/* Last error message of the handle; with h == NULL, of the last failed create
+ * on this thread. */
+const char *kifmm_last_error(kifmm_handle h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
