@@ -1,0 +1,179 @@
+"""Benchmark: one warm KIFMM3d evaluate on one MI355X.
+
+Workload (default): 8,388,608 nonuniform points (90% from 16 Gaussians, 10%
+uniform), Laplace single layer, np 6, ptsMax 128, maxLevel 14; error against
+the FP64 direct sum at 4,096 sampled targets.  Points, tree, lists, operators
+and densities are resident in HBM before the timed region; one step is one
+evaluate (device pointers in, device pointers out).
+
+Prints one JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fmm-g_amd"))
+
+METRIC = "Mpoints/s per FMM eval at ≤1e-6 rel err, mixed vs FP64 GPU; % roofline"
+HBM_PEAK_GBS = 8000.0
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+
+CONFIGS = {
+    # name: (kernel, coefs, np, ptsMax, maxLevel, generator, n)
+    "c1": (111, [1.0, 0.25], 4, 64, 30, "uniform", 32768),
+    "c3": (111, [1.0, 0.25], 6, 128, 14, "gaussians", 8388608),
+}
+WORKLOAD_NAME = {
+    "c1": "Laplace-S uniform N=32768 np=4 ptsMax=64",
+    "c3": "Laplace-S nonuniform (16 Gaussians + 10% uniform) N=8388608 np=6 ptsMax=128 maxLevel=14",
+}
+
+
+def make_points(gen, n, cfg_index):
+    from kifmm import workloads as W
+    if gen == "uniform":
+        return W.uniform(n, W.seed(cfg_index, 0)), None
+    if gen == "gaussians":
+        return W.gaussians(n, W.seed(cfg_index, 0)), None
+    raise ValueError(gen)
+
+
+def cpu_baseline(cfg, precision_ops, n_sample, cfg_index):
+    """The CPU oracle (reference algorithm restated in C, 1 thread) on a bounded
+    sample of the same distribution; returns Mpoints/s and the sample size."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from kifmm import workloads as W
+    kt, coefs, np_, pm, ml, gen, _ = cfg
+    pos, _ = make_points(gen, n_sample, cfg_index)
+    den = W.densities(n_sample, 1, W.seed(cfg_index, 1))
+    pu, pd = precision_ops
+    ops = O.Ops(O.mm_kernel(kt), coefs, np_, pu, pd)
+    tree = O.Tree(pos, pts_max=pm, max_level=ml)
+    t0 = time.perf_counter()
+    O.evaluate(tree, ops, kt, coefs, pos, np.zeros_like(pos), pos, den)
+    dt = time.perf_counter() - t0
+    return n_sample / dt / 1e6, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--n", type=int, default=0, help="override the point count (diagnostics only)")
+    ap.add_argument("--cpu-sample", type=int, default=65536)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--numchk", type=int, default=4096)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on exactly one GPU (multi-GPU sharding is a design, "
+              "not a build); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import kifmm
+    from kifmm import workloads as W
+
+    cfg = CONFIGS[args.config]
+    cfg_index = {"c1": 0, "c3": 2}[args.config]
+    kt, coefs, np_, pm, ml, gen, n = cfg
+    if args.n:
+        n = args.n
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    pos, nor = make_points(gen, n, cfg_index)
+    den = W.densities(n, 1, W.seed(cfg_index, 1))
+    f = kifmm.FMM3d(kt, coefs, np_, pm, ml, precision=args.precision)
+    f.set_points(pos, nor)
+    f.setup()
+    t_setup = time.perf_counter() - t0
+    print("setup %.2fs n=%d" % (t_setup, n), file=sys.stderr, flush=True)
+    d_den = torch.from_numpy(den).to(dev)
+    d_val = torch.empty(n * f.trg_dof, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    for _ in range(args.warmup):
+        f.evaluate_device(d_den, d_val)
+    torch.cuda.synchronize()
+
+    f.set_timing(True)
+    phase_sum = {}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        f.evaluate_device(d_den, d_val)
+        for k, v in f.phase_times().items():
+            phase_sum[k] = phase_sum.get(k, 0.0) + v
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    f.set_timing(False)
+    phases = {k: v / args.steps for k, v in phase_sum.items()}
+    print("ms/step %.3f phases %s" % (dt * 1e3, json.dumps({k: round(v, 3) for k, v in phases.items()})),
+          file=sys.stderr, flush=True)
+
+    # accuracy vs the FP64 direct sum at sampled targets (fmm3d_check_mpi.cpp:159-165)
+    rng = np.random.default_rng(W.seed(cfg_index, 3))
+    idx = rng.choice(n, size=min(args.numchk, n), replace=False)
+    d_chk = torch.from_numpy(np.ascontiguousarray(pos[idx])).to(dev)
+    d_ref = torch.empty(len(idx) * f.trg_dof, dtype=torch.float64, device=dev)
+    f.direct_device(d_den, d_chk, d_ref)
+    torch.cuda.synchronize()
+    ref = d_ref.cpu().numpy()
+    got = d_val.cpu().numpy()[idx]
+    rel_err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+
+    # roofline of the dominant kernel (near field: U/W/X S2T, M2T, L2T)
+    w = f.work_counts()
+    near_ms = phases["EvlUList"]
+    flops_per_pair = 11.0  # Laplace: 3 sub, 3 mul + 2 add (r^2), 1 rsqrt/div, 2 for the FMA
+    achieved = w["near_pairs"] * flops_per_pair / (near_ms * 1e-3) / 1e12
+    peak = PEAK_TFLOPS["fp64" if args.precision == "fp64" else "fp32"]
+    roofline = {"bound": "valu", "kernel": "k_near", "achieved": round(achieved, 3), "peak": peak,
+                "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+                "kernel_ms": round(near_ms, 4)}
+
+    cpu = None
+    if not args.no_cpu:
+        nue = np_ ** 3 - (np_ - 2) ** 3
+        nuc = (np_ + 2) ** 3 - np_ ** 3
+        pu = f.operator(0).reshape(nuc, nue).T
+        pd = f.operator(1).reshape(nue, nue).T
+        sample = min(args.cpu_sample, n)
+        mps, secs = cpu_baseline(cfg, (pu, pd), sample, cfg_index)
+        cpu = {"value": round(mps, 5), "unit": "Mpoints/s", "cores": 1, "kind": "port",
+               "sample": "%d points of the same distribution, one evaluate, %.1f s" % (sample, secs)}
+
+    out = {
+        "metric": METRIC,
+        "value": round(n / dt / 1e6, 3),
+        "unit": "Mpoints/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": WORKLOAD_NAME[args.config] if not args.n else "%s (n=%d)" % (args.config, n),
+                   "kernel": kt, "np": np_, "ptsMax": pm, "maxLevel": ml, "precision": args.precision,
+                   "n_points": n},
+        "rel_err": rel_err,
+        "phases_ms": {k: round(v, 4) for k, v in phases.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

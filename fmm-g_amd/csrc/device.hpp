@@ -63,7 +63,7 @@ struct VlistArgs {
   const double2* twiddle;      // n entries exp(-2 pi i m/n)
   const double* UE; int sz_ue;
   double* DC; int sz_dc;
-  const int* halo; const uint64_t* mask; const int* tgt;
+  const int* halo; const int* hord; const uint64_t* mask; const int* tgt;
   const double2* m2l;          // [343][sd*td][H]
   double2* dspec;              // [nsrc][sd][H]
   double2* espec;              // [ntrg][td][H]

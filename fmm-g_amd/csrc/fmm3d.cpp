@@ -219,7 +219,8 @@ int FMM3d::build_tasks() {
   }
 
   // --- V list as sibling groups ---
-  vsrc_node_h_.clear(); vtrg_node_h_.clear(); vhalo_h_.clear(); vtgt_h_.clear(); vmask_h_.clear();
+  vsrc_node_h_.clear(); vtrg_node_h_.clear(); vhalo_h_.clear(); vhord_h_.clear(); vtgt_h_.clear();
+  vmask_h_.clear();
   std::vector<int> src_slot(N, -1);
   std::map<int, int> grp;
   for (int B = 0; B < N; B++) {
@@ -253,6 +254,19 @@ int FMM3d::build_tasks() {
       vmask_h_[size_t(g) * 32 + tau * 4 + (r >> 6)] |= (uint64_t(1) << (r & 63));
       work_.vlist_pairs += 1;
     }
+  }
+  // per group: halo slots ordered by node index (the V lists' order)
+  const int ngrp = int(grp.size());
+  vhord_h_.assign(size_t(ngrp) * 216, -1);
+  for (auto& pg : grp) {
+    const int g = pg.second;
+    std::vector<std::pair<int, int>> nodes;
+    for (int r = 0; r < 216; r++) {
+      int sl = vhalo_h_[size_t(g) * 216 + r];
+      if (sl >= 0) nodes.push_back({vsrc_node_h_[sl], r});
+    }
+    std::sort(nodes.begin(), nodes.end());
+    for (size_t k = 0; k < nodes.size(); k++) vhord_h_[size_t(g) * 216 + k] = nodes[k].second;
   }
   work_.n_vsrc = double(vsrc_node_h_.size());
   work_.n_vtrg = double(vtrg_node_h_.size());
@@ -337,6 +351,7 @@ int FMM3d::upload_state() {
   KIFMM_UP(vsrc_d_, vsrc_node_h_, "upload V sources");
   KIFMM_UP(vtrg_d_, vtrg_node_h_, "upload V targets");
   KIFMM_UP(vhalo_d_, vhalo_h_, "upload V halo");
+  KIFMM_UP(vhord_d_, vhord_h_, "upload V order");
   KIFMM_UP(vtgt_d_, vtgt_h_, "upload V groups");
   KIFMM_UP(vmask_d_, vmask_h_, "upload V masks");
   if ((rc = check(dspec_d_.alloc(sizeof(double2) * vsrc_node_h_.size() * op.sd * op.H), "alloc V spectra"))) return rc;
@@ -424,7 +439,7 @@ int FMM3d::evaluate(const double* d_src_den, double* d_trg_val, hipStream_t s) {
   v.deg = deg_d_.as<double>(); v.shell = shell_d_.as<int>(); v.twiddle = twiddle_d_.as<double2>();
   v.UE = ue_d_.as<double>(); v.sz_ue = op.dat(UE);
   v.DC = dc_d_.as<double>(); v.sz_dc = op.dat(DC);
-  v.halo = vhalo_d_.as<int>(); v.mask = vmask_d_.as<uint64_t>(); v.tgt = vtgt_d_.as<int>();
+  v.halo = vhalo_d_.as<int>(); v.hord = vhord_d_.as<int>(); v.mask = vmask_d_.as<uint64_t>(); v.tgt = vtgt_d_.as<int>();
   v.m2l = m2l_d_.as<double2>(); v.dspec = dspec_d_.as<double2>(); v.espec = espec_d_.as<double2>();
   if ((rc = check(launch_vlist(v, s), "V list"))) return rc;
 

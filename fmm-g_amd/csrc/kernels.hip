@@ -168,7 +168,8 @@ __global__ __launch_bounds__(kSurfThreads) void k_surface(SurfArgs a, KParams<do
     tp[u][0] = g.x + g.w * a.sam[3 * qq];
     tp[u][1] = g.y + g.w * a.sam[3 * qq + 1];
     tp[u][2] = g.z + g.w * a.sam[3 * qq + 2];
-    for (int t = 0; t < TD; t++) acc[u][t] = 0.0;
+    // start from the current check value and add term by term (dgemv order)
+    for (int t = 0; t < TD; t++) acc[u][t] = act[u] ? a.out[int64_t(tk.node) * a.sz_out + q * TD + t] : 0.0;
   }
   for (int it = tk.ibeg; it < tk.iend; it++) {
     const int node = a.items[it];
@@ -201,7 +202,7 @@ __global__ __launch_bounds__(kSurfThreads) void k_surface(SurfArgs a, KParams<do
   for (int u = 0; u < 2; u++) {
     if (!act[u]) continue;
     const int q = lane + u * kSurfThreads;
-    for (int t = 0; t < TD; t++) o[q * TD + t] += acc[u][t];
+    for (int t = 0; t < TD; t++) o[q * TD + t] = acc[u][t];
   }
 }
 
@@ -222,11 +223,18 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
     const int j = n0 + tid;
     bcol[tid] = j < g.ncol ? int64_t(g.bidx[j]) * g.ldb : int64_t(-1);
   }
+  // Accumulating steps start from the current C entry and add term by term
+  // (the order of dgemv(1.0, A, x, 1.0, y)); overwriting steps start from 0.
   double c[4][4];
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+  for (int j = 0; j < 4; j++) {
+    const int n = n0 + tx + 16 * j;
 #pragma unroll
-    for (int j = 0; j < 4; j++) c[i][j] = 0.0;
+    for (int i = 0; i < 4; i++) {
+      const int m = m0 + ty + 16 * i;
+      c[i][j] = (g.accumulate && n < g.ncol && m < g.M) ? g.C[int64_t(g.cidx[n]) * g.ldc + m] : 0.0;
+    }
+  }
   __syncthreads();
   for (int k0 = 0; k0 < g.K; k0 += GT_K) {
     for (int e = tid; e < GT_M * GT_K; e += 256) {
@@ -264,8 +272,7 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
     for (int i = 0; i < 4; i++) {
       const int m = m0 + ty + 16 * i;
       if (m >= g.M) continue;
-      const double v = g.rscale[m % g.rsd] * c[i][j];
-      col[m] = g.accumulate ? col[m] + v : v;
+      col[m] = g.accumulate ? c[i][j] : g.rscale[m % g.rsd] * c[i][j];
     }
   }
 }
@@ -361,9 +368,13 @@ __global__ __launch_bounds__(64) void k_hadamard(VlistArgs v, double nrm) {
     for (int i = 0; i < TD; i++) acc[t][i] = make_double2(0, 0);
   const uint64_t* mk = v.mask + int64_t(g) * 32;
   const int* halo = v.halo + int64_t(g) * 216;
-  for (int r = 0; r < 216; r++) {
+  const int* hord = v.hord + int64_t(g) * 216;
+  // Halo slots in ascending node index: every target's V list is a subsequence
+  // of this order, so each accumulator sees its terms in V-list order.
+  for (int k = 0; k < 216; k++) {
+    const int r = hord[k];
+    if (r < 0) break;
     const int s = halo[r];
-    if (s < 0) continue;
     double2 d[SD];
 #pragma unroll
     for (int j = 0; j < SD; j++) d[j] = v.dspec[(int64_t(s) * SD + j) * H + ff];
@@ -377,7 +388,10 @@ __global__ __launch_bounds__(64) void k_hadamard(VlistArgs v, double nrm) {
 #pragma unroll
       for (int i = 0; i < TD; i++)
 #pragma unroll
-        for (int j = 0; j < SD; j++) acc[tau][i] = cadd(acc[tau][i], cmul(T[int64_t(j * TD + i) * H + ff], d[j]));
+        for (int j = 0; j < SD; j++) {
+          const double2 p = cmul(T[int64_t(j * TD + i) * H + ff], d[j]);
+          acc[tau][i] = make_double2(acc[tau][i].x + nrm * p.x, acc[tau][i].y + nrm * p.y);
+        }
     }
   }
   if (!ok) return;
@@ -387,7 +401,7 @@ __global__ __launch_bounds__(64) void k_hadamard(VlistArgs v, double nrm) {
     if (t < 0) continue;
 #pragma unroll
     for (int i = 0; i < TD; i++)
-      v.espec[(int64_t(t) * TD + i) * H + f] = make_double2(acc[tau][i].x * nrm, acc[tau][i].y * nrm);
+      v.espec[(int64_t(t) * TD + i) * H + f] = acc[tau][i];
   }
 }
 

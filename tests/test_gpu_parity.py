@@ -1,0 +1,152 @@
+"""GPU parity: the HIP evaluate pass through the C ABI against the CPU oracle.
+
+Bar (FP64 path): relative 2-norm difference over all targets <= 1e-12 against the
+oracle's restatement of the reference CPU evaluate (fmm3d_eval_mpi.cpp:396-1515)
+run on the same tree with the same pseudo-inverse tables; plus the reference's
+own accuracy check against the direct sum (fmm3d_check_mpi.cpp:159-165).
+"""
+import numpy as np
+import pytest
+
+import kifmm
+import oracle as O
+from kifmm import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+PARITY = 1e-12
+
+
+def rel(a, b):
+    r = np.linalg.norm(a - b) / np.linalg.norm(b)
+    print("relative difference %.3e" % r, flush=True)
+    return r
+
+
+def run_pair(kt, coefs, np_, pts_max, pos, nor, den, max_level=30, tpos=None):
+    print("case kt=%d np=%d n=%d ptsMax=%d" % (kt, np_, len(pos), pts_max), flush=True)
+    f = kifmm.FMM3d(kt, coefs, np_, pts_max, max_level)
+    f.set_points(pos, nor, tpos)
+    f.setup()
+    val = f.evaluate(den)
+    km = O.mm_kernel(kt)
+    sd, td = O.src_dof(km), O.trg_dof(km)
+    nue = np_ ** 3 - (np_ - 2) ** 3
+    nuc = (np_ + 2) ** 3 - np_ ** 3
+    pu = f.operator(0).reshape(nuc * td, nue * sd).T
+    pd = f.operator(1).reshape(nue * td, nue * sd).T
+    mmc = list(coefs) if kt not in (312, 322) else [1.0, 0.25]
+    ops = O.Ops(km, mmc, np_, pu, pd)
+    tree = O.Tree(pos, tpos, pts_max=pts_max, max_level=max_level)
+    ref = O.evaluate(tree, ops, kt, coefs, pos, nor, pos if tpos is None else tpos, den)
+    return f, val, ref
+
+
+def test_laplace_uniform_32k():
+    """32,768 uniform points, Laplace single layer, np 4, ptsMax 64 (the CPU-runnable case)."""
+    n = 32768
+    pos = W.uniform(n, W.seed(0, 0))
+    den = W.densities(n, 1, W.seed(0, 1))
+    f, val, ref = run_pair(111, [1.0, 0.25], 4, 64, pos, np.zeros_like(pos), den)
+    assert rel(val, ref) <= PARITY
+    err = f.check(den, val, 512, seed=W.seed(0, 3))
+    assert err < 5e-4, err
+
+
+@pytest.mark.parametrize("kt,coefs", [(121, [1.0, 0.25]), (311, [1.0, 0.25]), (312, [1.0, 0.25]),
+                                      (321, [1.0, 0.25]), (322, [1.0, 0.25]), (511, [1.0, 0.3]),
+                                      (521, [1.0, 0.3])])
+def test_kernels_spheres(kt, coefs):
+    """Every supported user kernel on adaptive sphere-surface points (W and X lists active)."""
+    n = 6000
+    pos, nor = W.spheres(n, 7 + kt, nsph=6)
+    sd = O.src_dof(kt)
+    den = W.densities(n, sd, 11 + kt, kind="normal")
+    f, val, ref = run_pair(kt, coefs, 4, 24, pos, nor, den)
+    assert rel(val, ref) <= PARITY
+
+
+def test_laplace_np6_gaussians():
+    """Adaptive Gaussian clusters with a depth cap, np 6 (all list types and the depth cap)."""
+    n = 40000
+    pos = W.gaussians(n, 99)
+    den = W.densities(n, 1, 100)
+    f, val, ref = run_pair(111, [1.0, 0.25], 6, 32, pos, np.zeros_like(pos), den, max_level=7)
+    assert rel(val, ref) <= PARITY
+    err = f.check(den, val, 1024, seed=5)
+    assert err < 1e-5, err
+
+
+def test_navier_np8():
+    n = 5000
+    pos = W.uniform(n, 31)
+    den = W.densities(n, 3, 32, kind="normal")
+    f, val, ref = run_pair(511, [1.0, 0.3], 8, 60, pos, np.zeros_like(pos), den, max_level=10)
+    assert rel(val, ref) <= PARITY
+
+
+def test_separate_targets():
+    """Targets distinct from sources (trgData on its own point set)."""
+    pos = W.uniform(8000, 41)
+    tpos = W.uniform(3000, 42)
+    den = W.densities(8000, 1, 43)
+    f, val, ref = run_pair(111, [1.0, 0.25], 4, 32, pos, np.zeros_like(pos), den, tpos=tpos)
+    assert rel(val, ref) <= PARITY
+
+
+def test_known_answer_single_level():
+    """N <= ptsMax: the root split is forced, every leaf's U list holds all 8
+    siblings, so the FMM equals the direct sum to round-off."""
+    n = 50
+    pos = W.uniform(n, 51)
+    den = W.densities(n, 1, 52)
+    f = kifmm.FMM3d(111, [1.0, 0.25], 4, 64, 30)
+    f.set_points(pos)
+    f.setup()
+    val = f.evaluate(den)
+    d = O.direct(111, [1.0, 0.25], pos, np.zeros_like(pos), den, pos)
+    assert rel(val, d) < 1e-14
+
+
+def test_empty_leaves_and_duplicates():
+    """Coincident points (r < 1e-8 excluded) and empty octants."""
+    base = W.uniform(300, 61)
+    pos = np.concatenate([base, base[:40], np.full((5, 3), 0.123)])
+    den = W.densities(len(pos), 1, 62)
+    f, val, ref = run_pair(111, [1.0, 0.25], 4, 8, pos, np.zeros_like(pos), den)
+    assert rel(val, ref) <= PARITY
+
+
+def test_repeat_evaluate_and_linearity():
+    """evaluate() is repeatable (buffers re-zeroed) and linear in the densities.
+
+    Linearity holds to round-off times the amplification of the np=6
+    pseudo-inverses (cutoff 1e-7): the CPU oracle differs from itself by
+    1e-11..4e-11 when only its FMA contraction changes, so the bound is 1e-10."""
+    n = 20000
+    pos = W.uniform(n, 71)
+    f = kifmm.FMM3d(111, [1.0, 0.25], 6, 64, 30)
+    f.set_points(pos)
+    f.setup()
+    a, b = W.densities(n, 1, 72), W.densities(n, 1, 73)
+    va, vb = f.evaluate(a), f.evaluate(b)
+    vab = f.evaluate(2.0 * a - 0.5 * b)
+    assert np.array_equal(f.evaluate(a), va)
+    assert rel(vab, 2.0 * va - 0.5 * vb) < 1e-10
+
+
+def test_device_pointers(gpu):
+    import torch
+    n = 16384
+    pos = W.uniform(n, 81)
+    den = W.densities(n, 1, 82)
+    f = kifmm.FMM3d(111, [1.0, 0.25], 4, 64, 30)
+    f.set_points(pos)
+    f.setup()
+    host = f.evaluate(den)
+    d = torch.from_numpy(den).to(gpu)
+    v = torch.empty(n, dtype=torch.float64, device=gpu)
+    torch.cuda.synchronize()
+    f.evaluate_device(d, v)
+    torch.cuda.synchronize()
+    assert np.array_equal(v.cpu().numpy(), host)
