@@ -803,6 +803,51 @@ void orc_tree_free(orc_tree *t) {
 }
 
 /* ======================================================================
+ * M2L through the regular grid (matmgnt3d_mpi.cpp:276-431).
+ * ====================================================================== */
+/* plnDen2EffDen :276-307: scale by 2^(l*deg), scatter onto the grid, r2c */
+void orc_eff_den(const orc_ops *op, int level, const double *ue, double *eff) {
+  int np = op->np, sd = op->sd, n = 2 * np, G = n * n * n;
+  int nue = op->n_ue;
+  double *tmp = (double *)malloc(sizeof(double) * nue * sd);
+  double *reg = (double *)calloc((size_t)G * sd, sizeof(double));
+  for (int q = 0; q < nue; q++)
+    for (int s = 0; s < sd; s++) tmp[q * sd + s] = ue[q * sd + s] * pow(2.0, level * op->deg[s]);
+  sam2reg(np, sd, tmp, reg);
+  orc_rfft3(n, sd, reg, eff);
+  free(tmp); free(reg);
+}
+
+/* UpwEqu2DwnChk_dgemv :411-430 with cptwvv :502-512; idx = offset of the source
+ * box from the target box in box widths */
+void orc_m2l_hadamard(const orc_ops *op, const int idx[3], const double *eff_den, double *eff_val) {
+  int np = op->np, sd = op->sd, td = op->td, n = 2 * np, H = (np + 1) * n * n;
+  double nrmfc = 1.0 / (double)(n * n * n);
+  int slot = (idx[0] + 3) * 49 + (idx[1] + 3) * 7 + (idx[2] + 3);
+  const double *mat = op->m2l + (size_t)slot * sd * td * H * 2;
+  for (int it = 0; it < td; it++)
+    for (int js = 0; js < sd; js++) {
+      int moff = js * td + it;
+      for (int f = 0; f < H; f++) {
+        const double *x = mat + 2 * ((size_t)f * sd * td + moff);
+        const double *y = eff_den + 2 * ((size_t)f * sd + js);
+        double *z = eff_val + 2 * ((size_t)f * td + it);
+        z[0] += nrmfc * (x[0] * y[0] - x[1] * y[1]);
+        z[1] += nrmfc * (x[0] * y[1] + x[1] * y[0]);
+      }
+    }
+}
+
+/* effVal2PlnVal :333-371: c2r then gather the surface samples, added into dc */
+void orc_eff_val(const orc_ops *op, const double *eff_val, double *dc) {
+  int np = op->np, td = op->td, n = 2 * np, G = n * n * n;
+  double *regv = (double *)malloc(sizeof(double) * (size_t)G * td);
+  orc_irfft3(n, td, eff_val, regv);
+  reg2sam(np, td, regv, dc);
+  free(regv);
+}
+
+/* ======================================================================
  * Evaluate: fmm3d_eval_mpi.cpp:396-1515, CPU branches, one rank.
  * ====================================================================== */
 static void node_center(const orc_tree *t, int i, const double ctr[3], double rad, double *c) {
@@ -908,12 +953,9 @@ int orc_evaluate(const orc_tree *t, const orc_ops *op, int kt_user, const double
 
   /* V list :1007-1059 */
   {
-    int n = 2 * np, G = n * n * n, H = (np + 1) * n * n;
+    int n = 2 * np, H = (np + 1) * n * n;
     double **eff_den = (double **)calloc(N, sizeof(double *));
-    double *reg = (double *)malloc(sizeof(double) * (size_t)G * (sd > td ? sd : td));
     double *effv = (double *)malloc(sizeof(double) * 2 * (size_t)H * td);
-    double *regv = (double *)malloc(sizeof(double) * (size_t)G * td);
-    double nrmfc = 1.0 / (double)G;
     for (int i = N - 1; i >= 0; i--) {
       if (!(t->tag[i] & ORC_EVTR)) continue;
       if (t->v_off[i + 1] == t->v_off[i]) continue;
@@ -926,37 +968,16 @@ int orc_evaluate(const orc_tree *t, const orc_ops *op, int kt_user, const double
         double ca[3]; node_center(t, A, ctr, rad0, ca);
         int idx[3];
         for (int d = 0; d < 3; d++) idx[d] = (int)floor((ca[d] - ci[d]) / D + 0.5);
-        if (!eff_den[A]) { /* plnDen2EffDen :276-307 */
-          memset(reg, 0, sizeof(double) * (size_t)G * sd);
-          for (int qq = 0; qq < nUE; qq++)
-            for (int s = 0; s < sd; s++)
-              tmpd[qq * sd + s] = UE[(size_t)A * szUE + qq * sd + s] * pow(2.0, l * op->deg[s]);
-          sam2reg(np, sd, tmpd, reg);
+        if (!eff_den[A]) {
           eff_den[A] = (double *)malloc(sizeof(double) * 2 * (size_t)H * sd);
-          orc_rfft3(n, sd, reg, eff_den[A]);
+          orc_eff_den(op, l, UE + (size_t)A * szUE, eff_den[A]);
         }
-        /* UpwEqu2DwnChk_dgemv :411-430 + cptwvv :502-512 */
-        int slot = (idx[0] + 3) * 49 + (idx[1] + 3) * 7 + (idx[2] + 3);
-        const double *mat = op->m2l + (size_t)slot * sd * td * H * 2;
-        const double *dn = eff_den[A];
-        for (int it = 0; it < td; it++)
-          for (int js = 0; js < sd; js++) {
-            int moff = js * td + it;
-            for (int f = 0; f < H; f++) {
-              const double *x = mat + 2 * ((size_t)f * sd * td + moff);
-              const double *y = dn + 2 * ((size_t)f * sd + js);
-              double *z = effv + 2 * ((size_t)f * td + it);
-              z[0] += nrmfc * (x[0] * y[0] - x[1] * y[1]);
-              z[1] += nrmfc * (x[0] * y[1] + x[1] * y[0]);
-            }
-          }
+        orc_m2l_hadamard(op, idx, eff_den[A], effv);
       }
-      /* effVal2PlnVal :333-347 */
-      orc_irfft3(n, td, effv, regv);
-      reg2sam(np, td, regv, DC + (size_t)i * szDC);
+      orc_eff_val(op, effv, DC + (size_t)i * szDC);
     }
     for (int i = 0; i < N; i++) free(eff_den[i]);
-    free(eff_den); free(reg); free(effv); free(regv);
+    free(eff_den); free(effv);
   }
 
   /* W list :1062-1231 (CPU part) */

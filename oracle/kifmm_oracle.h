@@ -79,6 +79,13 @@ void orc_ops_free(orc_ops *op);
 int  orc_check_matrix(int kt_mm, const double *coefs, int ncoefs, int np, int which,
                       double *mat);
 
+/* M2L through the regular grid: spectrum of a box's UE at level `level`,
+ * Hadamard accumulation for one same-level pair at offset idx (source box minus
+ * target box, in box widths), and c2r + surface gather added into DC. */
+void orc_eff_den(const orc_ops *op, int level, const double *ue, double *eff);
+void orc_m2l_hadamard(const orc_ops *op, const int idx[3], const double *eff_den, double *eff_val);
+void orc_eff_val(const orc_ops *op, const double *eff_val, double *dc);
+
 /* evaluate ------------------------------------------------------------- */
 int  orc_evaluate(const orc_tree *t, const orc_ops *op, int kt_user,
                   const double *coefs, int root_level, const double ctr[3],

@@ -61,6 +61,9 @@ def lib():
         L.orc_check_matrix.argtypes = [C.c_int, P_D, C.c_int, C.c_int, C.c_int, P_D]
         L.orc_evaluate.argtypes = [C.POINTER(_Tree), C.POINTER(_Ops), C.c_int, P_D, C.c_int,
                                    P_D, P_D, P_D, P_D, P_D, P_D, P_D, P_D]
+        L.orc_eff_den.argtypes = [C.POINTER(_Ops), C.c_int, P_D, P_D]
+        L.orc_m2l_hadamard.argtypes = [C.POINTER(_Ops), C.POINTER(C.c_int), P_D, P_D]
+        L.orc_eff_val.argtypes = [C.POINTER(_Ops), P_D, P_D]
         L.orc_direct.argtypes = [C.c_int, P_D, C.c_int, P_D, P_D, P_D, C.c_int, P_D, P_D,
                                  C.c_int]
         _LIB = L
@@ -237,6 +240,21 @@ class Ops:
             lib().orc_ops_free(C.byref(self._o))
         except Exception:
             pass
+
+
+def m2l_fft(ops, idx, level, ue):
+    """One M2L through the grid: DC of a target box from the UE of the same-level
+    source box at offset idx (matmgnt3d_mpi.cpp:276-431)."""
+    H = (ops.np + 1) * (2 * ops.np) ** 2
+    ue = _f64(ue)
+    eff = np.zeros(2 * H * ops.sd)
+    val = np.zeros(2 * H * ops.td)
+    dc = np.zeros(ops.n_ue * ops.td)
+    lib().orc_eff_den(C.byref(ops._o), int(level), _p(ue), _p(eff))
+    ix = (C.c_int * 3)(*[int(v) for v in idx])
+    lib().orc_m2l_hadamard(C.byref(ops._o), ix, _p(eff), _p(val))
+    lib().orc_eff_val(C.byref(ops._o), _p(val), _p(dc))
+    return dc
 
 
 def evaluate(tree, ops, kt, coefs, spos, snor, tpos, den, root_level=1, ctr=(0.5, 0.5, 0.5),
