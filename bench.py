@@ -67,11 +67,14 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--n", type=int, default=0, help="override the point count (diagnostics only)")
-    ap.add_argument("--cpu-sample", type=int, default=65536)
+    ap.add_argument("--cpu-sample", type=int, default=131072)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--numchk", type=int, default=4096)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="copy 512 MiB device-to-device first (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU (multi-GPU sharding is a design, "
@@ -88,52 +91,67 @@ def main():
     if args.n:
         n = args.n
     dev = torch.device("cuda:0")
-    t0 = time.perf_counter()
     pos, nor = make_points(gen, n, cfg_index)
     den = W.densities(n, 1, W.seed(cfg_index, 1))
-    f = kifmm.FMM3d(kt, coefs, np_, pm, ml, precision=args.precision)
-    f.set_points(pos, nor)
-    f.setup()
-    t_setup = time.perf_counter() - t0
-    print("setup %.2fs n=%d" % (t_setup, n), file=sys.stderr, flush=True)
     d_den = torch.from_numpy(den).to(dev)
-    d_val = torch.empty(n * f.trg_dof, dtype=torch.float64, device=dev)
-    torch.cuda.synchronize()
-
-    for _ in range(args.warmup):
-        f.evaluate_device(d_den, d_val)
-    torch.cuda.synchronize()
-
-    f.set_timing(True)
-    phase_sum = {}
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        f.evaluate_device(d_den, d_val)
-        for k, v in f.phase_times().items():
-            phase_sum[k] = phase_sum.get(k, 0.0) + v
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / args.steps
-    f.set_timing(False)
-    phases = {k: v / args.steps for k, v in phase_sum.items()}
-    print("ms/step %.3f phases %s" % (dt * 1e3, json.dumps({k: round(v, 3) for k, v in phases.items()})),
-          file=sys.stderr, flush=True)
-
-    # accuracy vs the FP64 direct sum at sampled targets (fmm3d_check_mpi.cpp:159-165)
+    if args.calib_copy:
+        src = torch.ones(64 * 2 ** 20, dtype=torch.float64, device=dev)  # 512 MiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     rng = np.random.default_rng(W.seed(cfg_index, 3))
     idx = rng.choice(n, size=min(args.numchk, n), replace=False)
-    d_chk = torch.from_numpy(np.ascontiguousarray(pos[idx])).to(dev)
-    d_ref = torch.empty(len(idx) * f.trg_dof, dtype=torch.float64, device=dev)
-    f.direct_device(d_den, d_chk, d_ref)
-    torch.cuda.synchronize()
-    ref = d_ref.cpu().numpy()
-    got = d_val.cpu().numpy()[idx]
-    rel_err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+
+    def run(precision):
+        """Setup, warm-up, K timed evaluates; returns timings, phases and error."""
+        t0 = time.perf_counter()
+        f = kifmm.FMM3d(kt, coefs, np_, pm, ml, precision=precision)
+        f.set_points(pos, nor)
+        f.setup()
+        print("[%s] setup %.2fs n=%d" % (precision, time.perf_counter() - t0, n), file=sys.stderr, flush=True)
+        d_val = torch.empty(n * f.trg_dof, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        for _ in range(args.warmup):
+            f.evaluate_device(d_den, d_val)
+        torch.cuda.synchronize()
+        f.set_timing(True)
+        phase_sum = {}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            f.evaluate_device(d_den, d_val)
+            for k, v in f.phase_times().items():
+                phase_sum[k] = phase_sum.get(k, 0.0) + v
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        f.set_timing(False)
+        phases = {k: v / args.steps for k, v in phase_sum.items()}
+        # accuracy vs the FP64 direct sum at sampled targets (fmm3d_check_mpi.cpp:159-165)
+        d_chk = torch.from_numpy(np.ascontiguousarray(pos[idx])).to(dev)
+        d_ref = torch.empty(len(idx) * f.trg_dof, dtype=torch.float64, device=dev)
+        f.direct_device(d_den, d_chk, d_ref)
+        torch.cuda.synchronize()
+        ref = d_ref.cpu().numpy()
+        val = d_val.cpu().numpy()
+        rel_err = float(np.linalg.norm(val[idx] - ref) / np.linalg.norm(ref))
+        print("[%s] ms/step %.3f rel_err %.3e phases %s" % (precision, dt * 1e3, rel_err,
+              json.dumps({k: round(v, 3) for k, v in phases.items()})), file=sys.stderr, flush=True)
+        return f, dt, phases, rel_err, val
+
+    f64 = dt64 = None
+    if args.precision == "mixed" and not args.no_fp64:
+        f64, dt64, ph64, err64, val64 = run("fp64")
+        del f64
+    f, dt, phases, rel_err, val = run(args.precision)
+    diff_fp64 = None
+    if dt64 is not None:
+        diff_fp64 = float(np.linalg.norm(val - val64) / np.linalg.norm(val64))
 
     # roofline of the dominant kernel (near field: U/W/X S2T, M2T, L2T)
     w = f.work_counts()
     near_ms = phases["EvlUList"]
-    flops_per_pair = 11.0  # Laplace: 3 sub, 3 mul + 2 add (r^2), 1 rsqrt/div, 2 for the FMA
+    flops_per_pair = 11.0  # Laplace: 3 sub, 3 mul + 2 add (r^2), 1 rsqrt, 2 for the FMA
     achieved = w["near_pairs"] * flops_per_pair / (near_ms * 1e-3) / 1e12
     peak = PEAK_TFLOPS["fp64" if args.precision == "fp64" else "fp32"]
     roofline = {"bound": "valu", "kernel": "k_near", "achieved": round(achieved, 3), "peak": peak,
@@ -168,6 +186,9 @@ def main():
                    "kernel": kt, "np": np_, "ptsMax": pm, "maxLevel": ml, "precision": args.precision,
                    "n_points": n},
         "rel_err": rel_err,
+        "fp64_ms_per_step": round(dt64 * 1e3, 4) if dt64 else None,
+        "speedup_vs_fp64": round(dt64 / dt, 3) if dt64 else None,
+        "rel_diff_vs_fp64": diff_fp64,
         "phases_ms": {k: round(v, 4) for k, v in phases.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -15,6 +15,7 @@ struct NearChunk {
   int node, tbeg, tnum, ibeg, iend;
 };
 constexpr int kNearChunk = 128;
+constexpr int kPairStride = 224;  // 216 halo slots rounded up to a multiple of 8
 // Source item code: (node << 2) | kind
 enum ItemKind { ITEM_EXACT = 0, ITEM_UE = 1, ITEM_DE = 2 };
 
@@ -64,6 +65,9 @@ struct VlistArgs {
   const double* UE; int sz_ue;
   double* DC; int sz_dc;
   const int* halo; const int* hord; const uint64_t* mask; const int* tgt;
+  // per group, the halo sources in V-list (node) order, padded to a multiple of 8
+  // with -1: source slot, and (halo index | offset slot of the halo index << 8)
+  const int* psrc; const int* pslot; const int* pcnt;
   const double2* m2l;          // [343][sd*td][H]
   double2* dspec;              // [nsrc][sd][H]
   double2* espec;              // [ntrg][td][H]
@@ -77,6 +81,16 @@ hipError_t launch_near(int kt, const NearArgs& a, const KParams<double>& pu, con
 hipError_t launch_surface(int kt, const SurfArgs& a, const KParams<double>& pu, hipStream_t s);
 hipError_t launch_gemm(const GemmArgs& g, hipStream_t s);
 hipError_t launch_vlist(const VlistArgs& v, hipStream_t s);
+// Mixed-precision path (kernels_mixed.hip) and FP32-storage FFTs.
+hipError_t launch_near_mixed(int kt, const NearArgs& a, const KParams<double>& pu, const KParams<double>& pm,
+                             hipStream_t s);
+hipError_t launch_surface_mixed(int kt, const SurfArgs& a, const KParams<double>& pu, hipStream_t s);
+hipError_t launch_hadamard_mixed(const VlistArgs& v, const float2* dspec, const float2* m2l, float2* espec,
+                                 hipStream_t s);
+hipError_t launch_gemm_fma(const GemmArgs& g, hipStream_t s);
+hipError_t launch_fft_fwd32(const VlistArgs& v, float2* dspec, hipStream_t s);
+hipError_t launch_fft_inv32(const VlistArgs& v, const float2* espec, hipStream_t s);
+
 hipError_t launch_direct(int kt, const KParams<double>& p, int64_t ns, const double* spos,
                          const double* snor, const double* sden, int nt, const double* tpos,
                          double* tval, hipStream_t s);

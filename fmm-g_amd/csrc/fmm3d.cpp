@@ -220,6 +220,7 @@ int FMM3d::build_tasks() {
 
   // --- V list as sibling groups ---
   vsrc_node_h_.clear(); vtrg_node_h_.clear(); vhalo_h_.clear(); vhord_h_.clear(); vtgt_h_.clear();
+  vpsrc_h_.clear(); vpslot_h_.clear(); vpcnt_h_.clear();
   vmask_h_.clear();
   std::vector<int> src_slot(N, -1);
   std::map<int, int> grp;
@@ -267,6 +268,23 @@ int FMM3d::build_tasks() {
     }
     std::sort(nodes.begin(), nodes.end());
     for (size_t k = 0; k < nodes.size(); k++) vhord_h_[size_t(g) * 216 + k] = nodes[k].second;
+  }
+  vpsrc_h_.assign(size_t(ngrp) * kPairStride, -1);
+  vpslot_h_.assign(size_t(ngrp) * kPairStride, 0);
+  vpcnt_h_.assign(ngrp, 0);
+  for (int g = 0; g < ngrp; g++) {
+    int k = 0;
+    for (; k < 216 && vhord_h_[size_t(g) * 216 + k] >= 0; k++) {
+      const int r = vhord_h_[size_t(g) * 216 + k];
+      const int rx = r / 36, ry = (r / 6) % 6, rz = r % 6;  // halo coordinate + 2
+      const int slot = (rx + 1) * 49 + (ry + 1) * 7 + (rz + 1);  // offset (r - 2) + 3
+      int tm = 0;  // targets of the group whose V list holds this source
+      for (int t = 0; t < 8; t++)
+        if ((vmask_h_[size_t(g) * 32 + t * 4 + (r >> 6)] >> (r & 63)) & 1) tm |= 1 << t;
+      vpsrc_h_[size_t(g) * kPairStride + k] = vhalo_h_[size_t(g) * 216 + r];
+      vpslot_h_[size_t(g) * kPairStride + k] = r | slot << 8 | tm << 20;
+    }
+    vpcnt_h_[g] = (k + 7) / 8 * 8;
   }
   work_.n_vsrc = double(vsrc_node_h_.size());
   work_.n_vtrg = double(vtrg_node_h_.size());
@@ -352,10 +370,22 @@ int FMM3d::upload_state() {
   KIFMM_UP(vtrg_d_, vtrg_node_h_, "upload V targets");
   KIFMM_UP(vhalo_d_, vhalo_h_, "upload V halo");
   KIFMM_UP(vhord_d_, vhord_h_, "upload V order");
+  KIFMM_UP(vpsrc_d_, vpsrc_h_, "upload V pairs");
+  KIFMM_UP(vpslot_d_, vpslot_h_, "upload V pairs");
+  KIFMM_UP(vpcnt_d_, vpcnt_h_, "upload V pairs");
   KIFMM_UP(vtgt_d_, vtgt_h_, "upload V groups");
   KIFMM_UP(vmask_d_, vmask_h_, "upload V masks");
-  if ((rc = check(dspec_d_.alloc(sizeof(double2) * vsrc_node_h_.size() * op.sd * op.H), "alloc V spectra"))) return rc;
-  if ((rc = check(espec_d_.alloc(sizeof(double2) * vtrg_node_h_.size() * op.td * op.H), "alloc V spectra"))) return rc;
+  if (precision_ == PREC_FP64) {
+    if ((rc = check(dspec_d_.alloc(sizeof(double2) * vsrc_node_h_.size() * op.sd * op.H), "alloc V spectra"))) return rc;
+    if ((rc = check(espec_d_.alloc(sizeof(double2) * vtrg_node_h_.size() * op.td * op.H), "alloc V spectra"))) return rc;
+    m2l32_d_.release(); dspec32_d_.release(); espec32_d_.release();
+  } else {
+    std::vector<float> m2l32(op.m2l.begin(), op.m2l.end());
+    KIFMM_UP(m2l32_d_, m2l32, "upload M2L (FP32)");
+    if ((rc = check(dspec32_d_.alloc(sizeof(float2) * vsrc_node_h_.size() * op.sd * op.H), "alloc V spectra"))) return rc;
+    if ((rc = check(espec32_d_.alloc(sizeof(float2) * vtrg_node_h_.size() * op.td * op.H), "alloc V spectra"))) return rc;
+    dspec_d_.release(); espec_d_.release();
+  }
 #undef KIFMM_UP
   if (events_.empty()) {
     events_.resize(PH_COUNT + 1);
@@ -378,6 +408,7 @@ int FMM3d::evaluate(const double* d_src_den, double* d_trg_val, hipStream_t s) {
   const KParams<double> pm = make_params(op.kt_mm, op.coefs.data());
   int rc;
   auto mark = [&](int ph) { if (timing_) (void)hipEventRecord(events_[ph], s); };
+  const bool mixed = precision_ == PREC_MIXED;
 
   mark(PH_INI);
   // zero the work vectors (:425-438)
@@ -415,7 +446,7 @@ int FMM3d::evaluate(const double* d_src_den, double* d_trg_val, hipStream_t s) {
       if (st.rs_off >= 0) { g.rscale = scales_d_.as<double>() + st.rs_off; g.rsd = op.sd; }
       else { g.rscale = ones_d_.as<double>(); g.rsd = 1; }
       g.ncol = st.ncol; g.accumulate = st.accumulate;
-      int r = check(launch_gemm(g, s), "level GEMM");
+      int r = check(mixed ? launch_gemm_fma(g, s) : launch_gemm(g, s), "level GEMM");
       if (r) return r;
     }
     return 0;
@@ -426,7 +457,7 @@ int FMM3d::evaluate(const double* d_src_den, double* d_trg_val, hipStream_t s) {
   sa.tasks = s2m_tasks_d_.as<SurfTask>(); sa.ntasks = int(s2m_tasks_h_.size());
   sa.sam = sam_d_[UC].as<double>(); sa.n_sam = op.n_sam[UC];
   sa.out = uc_d_.as<double>(); sa.sz_out = op.dat(UC);
-  if ((rc = check(launch_surface(kt_, sa, pu, s), "S2M"))) return rc;
+  if ((rc = check(mixed ? launch_surface_mixed(kt_, sa, pu, s) : launch_surface(kt_, sa, pu, s), "S2M"))) return rc;
   if ((rc = run_steps(up_steps_))) return rc;
 
   // V list (:1007-1059)
@@ -439,16 +470,24 @@ int FMM3d::evaluate(const double* d_src_den, double* d_trg_val, hipStream_t s) {
   v.deg = deg_d_.as<double>(); v.shell = shell_d_.as<int>(); v.twiddle = twiddle_d_.as<double2>();
   v.UE = ue_d_.as<double>(); v.sz_ue = op.dat(UE);
   v.DC = dc_d_.as<double>(); v.sz_dc = op.dat(DC);
-  v.halo = vhalo_d_.as<int>(); v.hord = vhord_d_.as<int>(); v.mask = vmask_d_.as<uint64_t>(); v.tgt = vtgt_d_.as<int>();
+  v.halo = vhalo_d_.as<int>(); v.hord = vhord_d_.as<int>();
+  v.psrc = vpsrc_d_.as<int>(); v.pslot = vpslot_d_.as<int>(); v.pcnt = vpcnt_d_.as<int>(); v.mask = vmask_d_.as<uint64_t>(); v.tgt = vtgt_d_.as<int>();
   v.m2l = m2l_d_.as<double2>(); v.dspec = dspec_d_.as<double2>(); v.espec = espec_d_.as<double2>();
-  if ((rc = check(launch_vlist(v, s), "V list"))) return rc;
+  if (!mixed) {
+    if ((rc = check(launch_vlist(v, s), "V list"))) return rc;
+  } else if (v.ngrp > 0) {
+    if ((rc = check(launch_fft_fwd32(v, dspec32_d_.as<float2>(), s), "V list forward"))) return rc;
+    if ((rc = check(launch_hadamard_mixed(v, dspec32_d_.as<float2>(), m2l32_d_.as<float2>(), espec32_d_.as<float2>(), s),
+                    "V list Hadamard"))) return rc;
+    if ((rc = check(launch_fft_inv32(v, espec32_d_.as<float2>(), s), "V list inverse"))) return rc;
+  }
 
   // X list S2L (:1233-1265)
   mark(PH_XLIST);
   sa.tasks = s2l_tasks_d_.as<SurfTask>(); sa.ntasks = int(s2l_tasks_h_.size());
   sa.sam = sam_d_[DC].as<double>(); sa.n_sam = op.n_sam[DC];
   sa.out = dc_d_.as<double>(); sa.sz_out = op.dat(DC);
-  if ((rc = check(launch_surface(kt_, sa, pu, s), "S2L"))) return rc;
+  if ((rc = check(mixed ? launch_surface_mixed(kt_, sa, pu, s) : launch_surface(kt_, sa, pu, s), "S2L"))) return rc;
 
   // down pass (:1267-1382)
   mark(PH_DOWN);
@@ -467,7 +506,8 @@ int FMM3d::evaluate(const double* d_src_den, double* d_trg_val, hipStream_t s) {
   na.UE = ue_d_.as<double>(); na.sz_ue = op.dat(UE);
   na.DE = de_d_.as<double>(); na.sz_de = op.dat(DE);
   na.tpos = tpos_d_.as<double>(); na.tval = tval_d_.as<double>();
-  if ((rc = check(launch_near(kt_, na, pu, pm, s), "near field"))) return rc;
+  if ((rc = check(mixed ? launch_near_mixed(kt_, na, pu, pm, s) : launch_near(kt_, na, pu, pm, s), "near field")))
+    return rc;
 
   // write back (:1490-1510)
   mark(PH_FIN);

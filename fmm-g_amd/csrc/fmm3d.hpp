@@ -124,7 +124,7 @@ class FMM3d {
   std::vector<int> near_items_h_;
   std::vector<SurfTask> s2m_tasks_h_, s2l_tasks_h_;
   std::vector<int> surf_items_h_;
-  std::vector<int> vsrc_node_h_, vtrg_node_h_, vhalo_h_, vhord_h_, vtgt_h_;
+  std::vector<int> vsrc_node_h_, vtrg_node_h_, vhalo_h_, vhord_h_, vtgt_h_, vpsrc_h_, vpslot_h_, vpcnt_h_;
   std::vector<uint64_t> vmask_h_;
 
   // device state
@@ -135,8 +135,9 @@ class FMM3d {
   DeviceBuffer lvl_idx_d_;
   DeviceBuffer sam_d_[4], shell_d_, twiddle_d_, deg_d_, ones_d_, scales_d_;
   DeviceBuffer pup_d_, pdn_d_, m2m_d_, l2l_d_, m2l_d_;
-  DeviceBuffer vsrc_d_, vtrg_d_, vhalo_d_, vhord_d_, vtgt_d_, vmask_d_, dspec_d_, espec_d_;
+  DeviceBuffer vsrc_d_, vtrg_d_, vhalo_d_, vhord_d_, vtgt_d_, vpsrc_d_, vpslot_d_, vpcnt_d_, vmask_d_, dspec_d_, espec_d_;
   DeviceBuffer direct_pos_d_, direct_nor_d_;
+  DeviceBuffer m2l32_d_, dspec32_d_, espec32_d_;  // mixed path: FP32 spectra
   std::vector<hipEvent_t> events_;
 };
 

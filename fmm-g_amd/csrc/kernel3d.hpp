@@ -52,6 +52,9 @@ inline int homogeneous_deg(int kt, double* deg) {
 }
 
 inline bool needs_normals(int kt) { return kt == 121 || kt == 321 || kt == 322 || kt == 521; }
+__host__ __device__ constexpr bool needs_normals_ct(int kt) {
+  return kt == 121 || kt == 321 || kt == 322 || kt == 521;
+}
 
 // Per-kernel constants, derived on the host in double from the coefficient
 // vector (mu, nu) exactly as the reference derives them.
@@ -87,10 +90,11 @@ inline KParams<double> make_params(int kt, const double* coefs) {
 
 template <typename T> __host__ __device__ inline T ksqrt(T v) { return std::sqrt(v); }
 
-// One (target, source) block, row-major TD x SD: blk[t*SD+s]; (x,y,z) = target - source.
+// One (target, source) block, row-major TD x SD: blk[t*SD+s]; (x,y,z) = target - source;
+// rn = (x,y,z).n, used by the double-layer kernels only.
 template <int KT, typename T>
-__host__ __device__ inline void kblock(const KParams<T>& p, T x, T y, T z, T nx, T ny, T nz,
-                                       T* blk) {
+__host__ __device__ inline void kblock_rn(const KParams<T>& p, T x, T y, T z, T nx, T ny, T nz, T rn,
+                                          T* blk) {
   constexpr int SD = src_dof(KT), TD = trg_dof(KT);
   T r2 = x * x + y * y + z * z;
   T r = ksqrt(r2);
@@ -98,7 +102,6 @@ __host__ __device__ inline void kblock(const KParams<T>& p, T x, T y, T z, T nx,
   if constexpr (KT == 111) {
     blk[0] = sing ? T(0) : p.a / r;
   } else if constexpr (KT == 121) {
-    T rn = x * nx + y * ny + z * nz;
     T r3 = r2 * r;
     blk[0] = sing ? T(0) : -p.a / r3 * rn;
   } else if constexpr (KT == 191) {
@@ -123,14 +126,12 @@ __host__ __device__ inline void kblock(const KParams<T>& p, T x, T y, T z, T nx,
     blk[0] = f * (p.a * x / r3); blk[1] = f * (p.a * y / r3); blk[2] = f * (p.a * z / r3);
     if (sing) { blk[0] = blk[1] = blk[2] = T(0); }
   } else if constexpr (KT == 321) {
-    T rn = x * nx + y * ny + z * nz;
     T r5 = r2 * r2 * r;
     T C = sing ? T(0) : -p.a / r5;
     blk[0] = C * rn * x * x; blk[1] = C * rn * x * y; blk[2] = C * rn * x * z;
     blk[3] = C * rn * y * x; blk[4] = C * rn * y * y; blk[5] = C * rn * y * z;
     blk[6] = C * rn * z * x; blk[7] = C * rn * z * y; blk[8] = C * rn * z * z;
   } else if constexpr (KT == 322) {
-    T rn = x * nx + y * ny + z * nz;
     T r3 = r2 * r, r5 = r3 * r2;
     blk[0] = p.a * (nx / r3 - 3 * rn * x / r5);
     blk[1] = p.a * (ny / r3 - 3 * rn * y / r5);
@@ -159,7 +160,6 @@ __host__ __device__ inline void kblock(const KParams<T>& p, T x, T y, T z, T nx,
     blk[3] = m * (H * y * x);     blk[4] = m * (G + H * y * y); blk[5] = m * (H * y * z);
     blk[6] = m * (H * z * x);     blk[7] = m * (H * z * y);     blk[8] = m * (G + H * z * z);
   } else if constexpr (KT == 521) {
-    T rn = x * nx + y * ny + z * nz;
     T r3 = r2 * r, r5 = r3 * r2;
     T A = p.a / r3, B = p.b / r3, C = p.c / r5;
     if (sing) { A = B = C = T(0); rn = T(0); }
@@ -175,6 +175,11 @@ __host__ __device__ inline void kblock(const KParams<T>& p, T x, T y, T z, T nx,
   } else {
     for (int i = 0; i < SD * TD; i++) blk[i] = T(0);
   }
+}
+
+template <int KT, typename T>
+__host__ __device__ inline void kblock(const KParams<T>& p, T x, T y, T z, T nx, T ny, T nz, T* blk) {
+  kblock_rn<KT, T>(p, x, y, z, nx, ny, nz, x * nx + y * ny + z * nz, blk);
 }
 
 // Runtime dispatch to the compile-time kernel id.

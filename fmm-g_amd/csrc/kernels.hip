@@ -55,24 +55,52 @@ hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// Near field: one workgroup per chunk of <= 128 targets of an evaluator leaf;
-// source items staged through LDS 128 at a time; one target per lane.
+// Near field: one workgroup per chunk of <= 128 targets of an evaluator leaf.
+// Source items are staged through LDS 128 at a time.  Leaves are small (a few
+// tens of targets), so the 128 lanes are split into S groups that take every
+// S-th source of a stage (S = 128 / targets, at most 16); the S partial sums of
+// a target are reduced through LDS at the end.  The near field feeds no
+// pseudo-inverse, so its summation order is free; for the Laplace kernel 1/r
+// comes from rsqrt.
 // ---------------------------------------------------------------------------
+template <int KT>
+__device__ inline void near_pair(const KParams<double>& p, double dx, double dy, double dz, double nx,
+                                 double ny, double nz, const double* den, double* acc) {
+  constexpr int SD = src_dof(KT), TD = trg_dof(KT);
+  if constexpr (KT == 111) {
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const double g = r2 < kMinDif * kMinDif ? 0.0 : p.a * rsqrt(r2);
+    acc[0] += g * den[0];
+  } else {
+    double blk[SD * TD];
+    kblock<KT, double>(p, dx, dy, dz, nx, ny, nz, blk);
+#pragma unroll
+    for (int t = 0; t < TD; t++)
+#pragma unroll
+      for (int s = 0; s < SD; s++) acc[t] += blk[t * SD + s] * den[s];
+  }
+}
+
 template <int KT>
 __global__ __launch_bounds__(kNearChunk) void k_near(NearArgs a, KParams<double> pu, KParams<double> pm) {
   constexpr int KM = mm_kernel(KT);
   constexpr int SD = src_dof(KT), TD = trg_dof(KT), sd = src_dof(KM);
   constexpr int MAXD = SD > sd ? SD : sd;
+  constexpr bool NOR = needs_normals_ct(KT);
   static_assert(trg_dof(KM) == TD, "user and equivalent kernels share the target DOF");
   __shared__ double sp[kNearChunk * 3];
-  __shared__ double sn[kNearChunk * 3];
+  __shared__ double sn[NOR ? kNearChunk * 3 : 1];
   __shared__ double sv[kNearChunk * MAXD];
+  __shared__ double red[kNearChunk * TD];
   const NearChunk ch = a.chunks[blockIdx.x];
   const int lane = threadIdx.x;
-  const bool active = lane < ch.tnum;
+  int S = kNearChunk / max(ch.tnum, 1);
+  S = S > 16 ? 16 : S;
+  const int ti = lane % ch.tnum, sub = lane / ch.tnum;
+  const bool active = sub < S;
   double tx = 0, ty = 0, tz = 0;
   if (active) {
-    const double* p = a.tpos + 3 * int64_t(ch.tbeg + lane);
+    const double* p = a.tpos + 3 * int64_t(ch.tbeg + ti);
     tx = p[0]; ty = p[1]; tz = p[2];
   }
   double acc[TD];
@@ -89,19 +117,17 @@ __global__ __launch_bounds__(kNearChunk) void k_near(NearArgs a, KParams<double>
         __syncthreads();
         if (lane < m) {
           const int64_t g = int64_t(b + base + lane);
-          for (int d = 0; d < 3; d++) { sp[3 * lane + d] = a.spos[3 * g + d]; sn[3 * lane + d] = a.snor[3 * g + d]; }
+          for (int d = 0; d < 3; d++) sp[3 * lane + d] = a.spos[3 * g + d];
+          if constexpr (NOR) for (int d = 0; d < 3; d++) sn[3 * lane + d] = a.snor[3 * g + d];
           for (int s = 0; s < SD; s++) sv[lane * SD + s] = a.sden[g * SD + s];
         }
         __syncthreads();
         if (active) {
-          for (int j = 0; j < m; j++) {
-            double blk[SD * TD];
-            kblock<KT, double>(pu, tx - sp[3 * j], ty - sp[3 * j + 1], tz - sp[3 * j + 2], sn[3 * j],
-                               sn[3 * j + 1], sn[3 * j + 2], blk);
-#pragma unroll
-            for (int t = 0; t < TD; t++)
-#pragma unroll
-              for (int s = 0; s < SD; s++) acc[t] += blk[t * SD + s] * sv[j * SD + s];
+          for (int j = sub; j < m; j += S) {
+            double nx = 0, ny = 0, nz = 0;
+            if constexpr (NOR) { nx = sn[3 * j]; ny = sn[3 * j + 1]; nz = sn[3 * j + 2]; }
+            near_pair<KT>(pu, tx - sp[3 * j], ty - sp[3 * j + 1], tz - sp[3 * j + 2], nx, ny, nz,
+                          &sv[j * SD], acc);
           }
         }
       }
@@ -123,20 +149,22 @@ __global__ __launch_bounds__(kNearChunk) void k_near(NearArgs a, KParams<double>
         }
         __syncthreads();
         if (active) {
-          for (int j = 0; j < m; j++) {
-            double blk[sd * TD];
+          for (int j = sub; j < m; j += S) {
             const double x = sp[3 * j], y = sp[3 * j + 1], z = sp[3 * j + 2];
-            kblock<KM, double>(pm, tx - x, ty - y, tz - z, x, y, z, blk);
-#pragma unroll
-            for (int t = 0; t < TD; t++)
-#pragma unroll
-              for (int s = 0; s < sd; s++) acc[t] += blk[t * sd + s] * sv[j * sd + s];
+            near_pair<KM>(pm, tx - x, ty - y, tz - z, x, y, z, &sv[j * sd], acc);
           }
         }
       }
     }
   }
-  if (active) {
+  // reduce the S partial sums of each target
+  __syncthreads();
+  if (active && sub > 0)
+    for (int t = 0; t < TD; t++) red[(lane - ch.tnum) * TD + t] = acc[t];
+  __syncthreads();
+  if (lane < ch.tnum) {
+    for (int u = 1; u < S; u++)
+      for (int t = 0; t < TD; t++) acc[t] += red[((u - 1) * ch.tnum + lane) * TD + t];
     double* o = a.tval + int64_t(ch.tbeg + lane) * TD;
 #pragma unroll
     for (int t = 0; t < TD; t++) o[t] = acc[t];
@@ -298,7 +326,14 @@ __device__ inline double2 cmul(double2 a, double2 b) {
 }
 __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 
-__global__ __launch_bounds__(kFftThreads) void k_fft_fwd(VlistArgs v) {
+__device__ inline void store_c(double2* p, double2 v) { *p = v; }
+__device__ inline void store_c(float2* p, double2 v) { *p = make_float2(float(v.x), float(v.y)); }
+__device__ inline double2 load_c(const double2* p) { return *p; }
+__device__ inline double2 load_c(const float2* p) { const float2 v = *p; return make_double2(v.x, v.y); }
+
+// C2 = storage type of the spectra (double2, or float2 for the mixed path).
+template <typename C2>
+__global__ __launch_bounds__(kFftThreads) void k_fft_fwd(VlistArgs v, C2* dspec) {
   __shared__ double xb[kMaxNp * kMaxNp * kMaxNp];
   __shared__ double2 t1[kMaxNp * kMaxNp * kMaxH];
   __shared__ double2 t2[kMaxNp * kMaxN * kMaxH];
@@ -339,21 +374,23 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_fwd(VlistArgs v) {
     }
     __syncthreads();
     // along z (a) -> global, frequency index (ka*n+kb)*h+kc
-    double2* out = v.dspec + (int64_t(slot) * v.sd + s) * H;
+    C2* out = dspec + (int64_t(slot) * v.sd + s) * H;
     for (int e = tid; e < H; e += kFftThreads) {
       const int kc = e % h, kb = (e / h) % n, ka = e / (h * n);
       double2 acc = make_double2(0, 0);
       for (int a = 0; a < np; a++) acc = cadd(acc, cmul(t2[(a * n + kb) * h + kc], w[((a + o) * ka) % n]));
-      out[e] = acc;
+      store_c(out + e, acc);
     }
   }
 }
 
-// Hadamard accumulation: one workgroup per sibling group (the 8 children of a
-// parent) and 64 frequencies; lane = frequency.  Sources are the children of
-// the parent's colleagues ("halo", 6x6x6 slots); membership of each
-// (target child, halo slot) pair in the V list is a bit mask built from the
-// CSR lists, so the pairs are exactly the reference's.
+// Hadamard accumulation: one workgroup (one wave) per sibling group (the 8
+// children of a parent) and 64 frequencies; lane = frequency, each lane holds
+// the 8 targets' accumulators.  The host lists the group's sources (children
+// of the parent's colleagues) in V-list (node) order with a packed word (halo
+// index, offset slot of the halo index, 8-bit target membership), so every
+// target sees its terms in V-list order, with the 1/G factor applied per term
+// (UpwEqu2DwnChk_dgemv + cptwvv).
 template <int SD, int TD>
 __global__ __launch_bounds__(64) void k_hadamard(VlistArgs v, double nrm) {
   const int g = blockIdx.x;
@@ -366,24 +403,20 @@ __global__ __launch_bounds__(64) void k_hadamard(VlistArgs v, double nrm) {
   for (int t = 0; t < 8; t++)
 #pragma unroll
     for (int i = 0; i < TD; i++) acc[t][i] = make_double2(0, 0);
-  const uint64_t* mk = v.mask + int64_t(g) * 32;
-  const int* halo = v.halo + int64_t(g) * 216;
-  const int* hord = v.hord + int64_t(g) * 216;
-  // Halo slots in ascending node index: every target's V list is a subsequence
-  // of this order, so each accumulator sees its terms in V-list order.
-  for (int k = 0; k < 216; k++) {
-    const int r = hord[k];
-    if (r < 0) break;
-    const int s = halo[r];
+  const int cnt = v.pcnt[g];
+  const int* ps = v.psrc + int64_t(g) * kPairStride;
+  const int* pr = v.pslot + int64_t(g) * kPairStride;
+  for (int k = 0; k < cnt; k++) {
+    const int s = ps[k], word = pr[k];
+    if (s < 0) continue;
     double2 d[SD];
 #pragma unroll
     for (int j = 0; j < SD; j++) d[j] = v.dspec[(int64_t(s) * SD + j) * H + ff];
-    const int rx = r / 36, ry = (r / 6) % 6, rz = r % 6;  // halo coordinate + 2
+    const int rs = (word >> 8) & 0x3ff, tm = word >> 20;
 #pragma unroll
     for (int tau = 0; tau < 8; tau++) {
-      if (!((mk[tau * 4 + (r >> 6)] >> (r & 63)) & 1ull)) continue;
-      const int ox = rx - 2 - (tau >> 2), oy = ry - 2 - ((tau >> 1) & 1), oz = rz - 2 - (tau & 1);
-      const int slot = (ox + 3) * 49 + (oy + 3) * 7 + (oz + 3);
+      if (!((tm >> tau) & 1)) continue;
+      const int slot = rs - ((tau >> 2) * 49 + ((tau >> 1) & 1) * 7 + (tau & 1));
       const double2* T = v.m2l + int64_t(slot) * SD * TD * H;
 #pragma unroll
       for (int i = 0; i < TD; i++)
@@ -400,14 +433,14 @@ __global__ __launch_bounds__(64) void k_hadamard(VlistArgs v, double nrm) {
     const int t = v.tgt[g * 8 + tau];
     if (t < 0) continue;
 #pragma unroll
-    for (int i = 0; i < TD; i++)
-      v.espec[(int64_t(t) * TD + i) * H + f] = acc[tau][i];
+    for (int i = 0; i < TD; i++) v.espec[(int64_t(t) * TD + i) * H + f] = acc[tau][i];
   }
 }
 
 // Inverse: unnormalised c2r of each target spectrum, evaluated only on the
 // surface samples, added into DC (effVal2PlnVal + regVal2SamVal).
-__global__ __launch_bounds__(kFftThreads) void k_fft_inv(VlistArgs v) {
+template <typename C2>
+__global__ __launch_bounds__(kFftThreads) void k_fft_inv(VlistArgs v, const C2* espec) {
   __shared__ double2 xs[kMaxN * kMaxN * kMaxH];
   __shared__ double2 y1[kMaxNp * kMaxN * kMaxH];
   __shared__ double2 y2[kMaxNp * kMaxNp * kMaxH];
@@ -420,8 +453,8 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_inv(VlistArgs v) {
   if (tid < n) w[tid] = make_double2(v.twiddle[tid].x, -v.twiddle[tid].y);  // e^{+2 pi i m/n}
   for (int i = 0; i < v.td; i++) {
     __syncthreads();
-    const double2* in = v.espec + (int64_t(slot) * v.td + i) * H;
-    for (int e = tid; e < H; e += kFftThreads) xs[e] = in[e];
+    const C2* in = espec + (int64_t(slot) * v.td + i) * H;
+    for (int e = tid; e < H; e += kFftThreads) xs[e] = load_c(in + e);
     __syncthreads();
     // along z (a), only block rows a' in [0,np)
     for (int e = tid; e < np * n * h; e += kFftThreads) {
@@ -459,7 +492,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_inv(VlistArgs v) {
 hipError_t launch_vlist(const VlistArgs& v, hipStream_t s) {
   if (v.ngrp <= 0) return hipSuccess;
   if (v.np > kMaxNp) return hipErrorInvalidValue;
-  k_fft_fwd<<<dim3(unsigned(v.nsrc)), dim3(kFftThreads), 0, s>>>(v);
+  k_fft_fwd<double2><<<dim3(unsigned(v.nsrc)), dim3(kFftThreads), 0, s>>>(v, v.dspec);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const double nrm = 1.0 / double(v.n * v.n * v.n);
@@ -470,7 +503,21 @@ hipError_t launch_vlist(const VlistArgs& v, hipStream_t s) {
   else return hipErrorInvalidValue;
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  k_fft_inv<<<dim3(unsigned(v.ntrg)), dim3(kFftThreads), 0, s>>>(v);
+  k_fft_inv<double2><<<dim3(unsigned(v.ntrg)), dim3(kFftThreads), 0, s>>>(v, v.espec);
+  return hipGetLastError();
+}
+
+hipError_t launch_fft_fwd32(const VlistArgs& v, float2* dspec, hipStream_t s) {
+  if (v.nsrc <= 0) return hipSuccess;
+  if (v.np > kMaxNp) return hipErrorInvalidValue;
+  k_fft_fwd<float2><<<dim3(unsigned(v.nsrc)), dim3(kFftThreads), 0, s>>>(v, dspec);
+  return hipGetLastError();
+}
+
+hipError_t launch_fft_inv32(const VlistArgs& v, const float2* espec, hipStream_t s) {
+  if (v.ntrg <= 0) return hipSuccess;
+  if (v.np > kMaxNp) return hipErrorInvalidValue;
+  k_fft_inv<float2><<<dim3(unsigned(v.ntrg)), dim3(kFftThreads), 0, s>>>(v, espec);
   return hipGetLastError();
 }
 

@@ -1,0 +1,392 @@
+// Mixed-precision kernels of the evaluate pass (gfx950).  Built with FP
+// contraction: no rounding-order contract applies here.
+//
+//   near field (U, W, X fallbacks, M2T, L2T)  FP32 kernel evaluation in
+//       coordinates relative to the target leaf centre, FP32 partial sums per
+//       staged source block, FP64 running sums; near and far (L2T) parts summed
+//       in FP64 at the end
+//   S2M / S2L                                 same scheme, FP64 check values
+//   V-list Hadamard                           FP32 spectra, FP32 FMA accumulation
+//   translations and pseudo-inverse applies   FP64 GEMM with FMA
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+
+namespace kifmm {
+
+static KParams<float> to_f32(const KParams<double>& p) { return KParams<float>{float(p.a), float(p.b), float(p.c)}; }
+
+// ---------------------------------------------------------------------------
+template <int KT>
+__device__ inline void pair_f32(const KParams<float>& p, float dx, float dy, float dz, float nx, float ny,
+                                float nz, const float* den, float* acc) {
+  constexpr int SD = src_dof(KT), TD = trg_dof(KT);
+  if constexpr (KT == 111) {
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const float g = r2 < float(kMinDif * kMinDif) ? 0.0f : p.a * rsqrtf(r2);
+    acc[0] = fmaf(g, den[0], acc[0]);
+  } else {
+    float blk[SD * TD];
+    kblock<KT, float>(p, dx, dy, dz, nx, ny, nz, blk);
+#pragma unroll
+    for (int t = 0; t < TD; t++)
+#pragma unroll
+      for (int s = 0; s < SD; s++) acc[t] = fmaf(blk[t * SD + s], den[s], acc[t]);
+  }
+}
+
+// Double-layer kernels: r and r.n from FP64 positions (r.n cancels for close
+// pairs on a smooth surface), the rest of the kernel in FP32.
+template <int KT>
+__device__ inline void pair_f32_dl(const KParams<float>& p, double dx, double dy, double dz, float nx, float ny,
+                                   float nz, const float* den, float* acc) {
+  constexpr int SD = src_dof(KT), TD = trg_dof(KT);
+  const float rn = float(dx * double(nx) + dy * double(ny) + dz * double(nz));
+  float blk[SD * TD];
+  kblock_rn<KT, float>(p, float(dx), float(dy), float(dz), nx, ny, nz, rn, blk);
+#pragma unroll
+  for (int t = 0; t < TD; t++)
+#pragma unroll
+    for (int s = 0; s < SD; s++) acc[t] = fmaf(blk[t * SD + s], den[s], acc[t]);
+}
+
+template <int KT>
+__global__ __launch_bounds__(kNearChunk) void k_near_mx(NearArgs a, KParams<float> pu, KParams<float> pm) {
+  constexpr int KM = mm_kernel(KT);
+  constexpr int SD = src_dof(KT), TD = trg_dof(KT), sd = src_dof(KM);
+  constexpr int MAXD = SD > sd ? SD : sd;
+  constexpr bool NOR = needs_normals_ct(KT);
+  __shared__ float sp[kNearChunk * 3];
+  __shared__ float sn[NOR ? kNearChunk * 3 : 1];
+  __shared__ double sp64[NOR ? kNearChunk * 3 : 1];
+  __shared__ float sv[kNearChunk * MAXD];
+  __shared__ double red[kNearChunk * TD * 2];
+  const NearChunk ch = a.chunks[blockIdx.x];
+  const double4 cb = a.geom[ch.node];
+  const int lane = threadIdx.x;
+  int S = kNearChunk / max(ch.tnum, 1);
+  S = S > 16 ? 16 : S;
+  const int ti = lane % ch.tnum, sub = lane / ch.tnum;
+  const bool active = sub < S;
+  float tx = 0, ty = 0, tz = 0;
+  double tx64 = 0, ty64 = 0, tz64 = 0;
+  if (active) {
+    const double* p = a.tpos + 3 * int64_t(ch.tbeg + ti);
+    tx64 = p[0]; ty64 = p[1]; tz64 = p[2];
+    tx = float(p[0] - cb.x); ty = float(p[1] - cb.y); tz = float(p[2] - cb.z);
+  }
+  double near[TD], far[TD];
+#pragma unroll
+  for (int t = 0; t < TD; t++) near[t] = far[t] = 0.0;
+
+  for (int it = ch.ibeg; it < ch.iend; it++) {
+    const int code = a.items[it];
+    const int node = code >> 2, kind = code & 3;
+    float acc[TD];
+#pragma unroll
+    for (int t = 0; t < TD; t++) acc[t] = 0.0f;
+    if (kind == ITEM_EXACT) {
+      const int b = a.src_beg[node], ns = a.src_num[node];
+      for (int base = 0; base < ns; base += kNearChunk) {
+        const int m = min(kNearChunk, ns - base);
+        __syncthreads();
+        if (lane < m) {
+          const int64_t g = int64_t(b + base + lane);
+          sp[3 * lane] = float(a.spos[3 * g] - cb.x);
+          sp[3 * lane + 1] = float(a.spos[3 * g + 1] - cb.y);
+          sp[3 * lane + 2] = float(a.spos[3 * g + 2] - cb.z);
+          if constexpr (NOR)
+            for (int d = 0; d < 3; d++) { sn[3 * lane + d] = float(a.snor[3 * g + d]); sp64[3 * lane + d] = a.spos[3 * g + d]; }
+          for (int s = 0; s < SD; s++) sv[lane * SD + s] = float(a.sden[g * SD + s]);
+        }
+        __syncthreads();
+        if (active) {
+          for (int j = sub; j < m; j += S) {
+            if constexpr (NOR) {
+              pair_f32_dl<KT>(pu, tx64 - sp64[3 * j], ty64 - sp64[3 * j + 1], tz64 - sp64[3 * j + 2], sn[3 * j],
+                              sn[3 * j + 1], sn[3 * j + 2], &sv[j * SD], acc);
+            } else {
+              pair_f32<KT>(pu, tx - sp[3 * j], ty - sp[3 * j + 1], tz - sp[3 * j + 2], 0.f, 0.f, 0.f, &sv[j * SD], acc);
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < TD; t++) near[t] += double(acc[t]);
+    } else {
+      const double4 g = a.geom[node];
+      const bool ue = kind == ITEM_UE;
+      const double* sam = ue ? a.sam_ue : a.sam_de;
+      const int ns = ue ? a.n_ue : a.n_de;
+      const double* den = ue ? a.UE + int64_t(node) * a.sz_ue : a.DE + int64_t(node) * a.sz_de;
+      for (int base = 0; base < ns; base += kNearChunk) {
+        const int m = min(kNearChunk, ns - base);
+        __syncthreads();
+        if (lane < m) {
+          const int q = base + lane;
+          sp[3 * lane] = float((g.x - cb.x) + g.w * sam[3 * q]);
+          sp[3 * lane + 1] = float((g.y - cb.y) + g.w * sam[3 * q + 1]);
+          sp[3 * lane + 2] = float((g.z - cb.z) + g.w * sam[3 * q + 2]);
+          for (int s = 0; s < sd; s++) sv[lane * sd + s] = float(den[q * sd + s]);
+        }
+        __syncthreads();
+        if (active) {
+          for (int j = sub; j < m; j += S) {
+            const float x = sp[3 * j], y = sp[3 * j + 1], z = sp[3 * j + 2];
+            pair_f32<KM>(pm, tx - x, ty - y, tz - z, x, y, z, &sv[j * sd], acc);
+          }
+        }
+      }
+      if (ue) {
+#pragma unroll
+        for (int t = 0; t < TD; t++) near[t] += double(acc[t]);
+      } else {
+#pragma unroll
+        for (int t = 0; t < TD; t++) far[t] += double(acc[t]);
+      }
+    }
+  }
+  __syncthreads();
+  if (active && sub > 0)
+    for (int t = 0; t < TD; t++) {
+      red[((lane - ch.tnum) * TD + t) * 2] = near[t];
+      red[((lane - ch.tnum) * TD + t) * 2 + 1] = far[t];
+    }
+  __syncthreads();
+  if (lane < ch.tnum) {
+    for (int u = 1; u < S; u++)
+      for (int t = 0; t < TD; t++) {
+        near[t] += red[(((u - 1) * ch.tnum + lane) * TD + t) * 2];
+        far[t] += red[(((u - 1) * ch.tnum + lane) * TD + t) * 2 + 1];
+      }
+    double* o = a.tval + int64_t(ch.tbeg + lane) * TD;
+#pragma unroll
+    for (int t = 0; t < TD; t++) o[t] = near[t] + far[t];
+  }
+}
+
+// ---------------------------------------------------------------------------
+constexpr int kSurfMx = 256;
+
+template <int KT>
+__global__ __launch_bounds__(kSurfMx) void k_surface_mx(SurfArgs a, KParams<float> pu) {
+  constexpr int SD = src_dof(KT), TD = trg_dof(KT);
+  constexpr bool NOR = needs_normals_ct(KT);
+  __shared__ float sp[kSurfMx * 3];
+  __shared__ float sn[NOR ? kSurfMx * 3 : 1];
+  __shared__ float sv[kSurfMx * SD];
+  const SurfTask tk = a.tasks[blockIdx.x];
+  const double4 g = a.geom[tk.node];
+  const int lane = threadIdx.x;
+  float tp[2][3];
+  double acc[2][TD];
+  bool act[2];
+  for (int u = 0; u < 2; u++) {
+    const int q = lane + u * kSurfMx;
+    act[u] = q < a.n_sam;
+    const int qq = act[u] ? q : 0;
+    for (int d = 0; d < 3; d++) tp[u][d] = float(g.w * a.sam[3 * qq + d]);
+    for (int t = 0; t < TD; t++) acc[u][t] = 0.0;
+  }
+  for (int it = tk.ibeg; it < tk.iend; it++) {
+    const int node = a.items[it];
+    const int b = a.src_beg[node], ns = a.src_num[node];
+    for (int base = 0; base < ns; base += kSurfMx) {
+      const int m = min(kSurfMx, ns - base);
+      __syncthreads();
+      if (lane < m) {
+        const int64_t gg = int64_t(b + base + lane);
+        sp[3 * lane] = float(a.spos[3 * gg] - g.x);
+        sp[3 * lane + 1] = float(a.spos[3 * gg + 1] - g.y);
+        sp[3 * lane + 2] = float(a.spos[3 * gg + 2] - g.z);
+        if constexpr (NOR) for (int d = 0; d < 3; d++) sn[3 * lane + d] = float(a.snor[3 * gg + d]);
+        for (int s = 0; s < SD; s++) sv[lane * SD + s] = float(a.sden[gg * SD + s]);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        if (!act[u]) continue;
+        float part[TD];
+        for (int t = 0; t < TD; t++) part[t] = 0.0f;
+        for (int j = 0; j < m; j++) {
+          float nx = 0, ny = 0, nz = 0;
+          if constexpr (NOR) { nx = sn[3 * j]; ny = sn[3 * j + 1]; nz = sn[3 * j + 2]; }
+          pair_f32<KT>(pu, tp[u][0] - sp[3 * j], tp[u][1] - sp[3 * j + 1], tp[u][2] - sp[3 * j + 2], nx, ny, nz,
+                       &sv[j * SD], part);
+        }
+        for (int t = 0; t < TD; t++) acc[u][t] += double(part[t]);
+      }
+    }
+  }
+  double* o = a.out + int64_t(tk.node) * a.sz_out;
+  for (int u = 0; u < 2; u++) {
+    if (!act[u]) continue;
+    const int q = lane + u * kSurfMx;
+    for (int t = 0; t < TD; t++) o[q * TD + t] += acc[u][t];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// V-list Hadamard with FP32 spectra: one wave per sibling group and 64
+// frequencies, lane = frequency, the 8 targets' accumulators per lane.
+template <int SD, int TD>
+__global__ __launch_bounds__(64) void k_hadamard_mx(VlistArgs v, const float2* __restrict__ dspec,
+                                                   const float2* __restrict__ m2l, float2* __restrict__ espec,
+                                                   float nrm) {
+  const int g = blockIdx.x;
+  const int f = blockIdx.y * 64 + threadIdx.x;
+  const int H = v.H;
+  const bool ok = f < H;
+  const int ff = ok ? f : 0;
+  float2 acc[8][TD];
+#pragma unroll
+  for (int t = 0; t < 8; t++)
+#pragma unroll
+    for (int i = 0; i < TD; i++) acc[t][i] = make_float2(0.f, 0.f);
+  const int cnt = v.pcnt[g];
+  const int* ps = v.psrc + int64_t(g) * kPairStride;
+  const int* pr = v.pslot + int64_t(g) * kPairStride;
+  for (int k = 0; k < cnt; k++) {
+    const int s = ps[k], word = pr[k];
+    if (s < 0) continue;
+    float2 d[SD];
+#pragma unroll
+    for (int j = 0; j < SD; j++) d[j] = dspec[(int64_t(s) * SD + j) * H + ff];
+    const int rs = (word >> 8) & 0x3ff, tm = word >> 20;
+#pragma unroll
+    for (int tau = 0; tau < 8; tau++) {
+      if (!((tm >> tau) & 1)) continue;
+      const int slot = rs - ((tau >> 2) * 49 + ((tau >> 1) & 1) * 7 + (tau & 1));
+      const float2* T = m2l + int64_t(slot) * SD * TD * H;
+#pragma unroll
+      for (int i = 0; i < TD; i++)
+#pragma unroll
+        for (int j = 0; j < SD; j++) {
+          const float2 t = T[int64_t(j * TD + i) * H + ff];
+          acc[tau][i].x = fmaf(t.x, d[j].x, fmaf(-t.y, d[j].y, acc[tau][i].x));
+          acc[tau][i].y = fmaf(t.x, d[j].y, fmaf(t.y, d[j].x, acc[tau][i].y));
+        }
+    }
+  }
+  if (!ok) return;
+#pragma unroll
+  for (int tau = 0; tau < 8; tau++) {
+    const int t = v.tgt[g * 8 + tau];
+    if (t < 0) continue;
+#pragma unroll
+    for (int i = 0; i < TD; i++)
+      espec[(int64_t(t) * TD + i) * H + f] = make_float2(acc[tau][i].x * nrm, acc[tau][i].y * nrm);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Indexed batched FP64 GEMM with FMA (same contract as k_gemm).
+constexpr int GM = 64, GN = 64, GK = 16;
+
+__global__ __launch_bounds__(256) void k_gemm_fma(GemmArgs g) {
+  __shared__ double As[GK][GM + 1];
+  __shared__ double Bs[GK][GN + 1];
+  __shared__ int64_t bcol[GN];
+  const int m0 = blockIdx.x * GM, n0 = blockIdx.y * GN;
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  if (tid < GN) {
+    const int j = n0 + tid;
+    bcol[tid] = j < g.ncol ? int64_t(g.bidx[j]) * g.ldb : int64_t(-1);
+  }
+  double c[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) c[i][j] = 0.0;
+  __syncthreads();
+  for (int k0 = 0; k0 < g.K; k0 += GK) {
+    for (int e = tid; e < GM * GK; e += 256) {
+      const int mm = e % GM, kk = e / GM;
+      const int m = m0 + mm, k = k0 + kk;
+      As[kk][mm] = (m < g.M && k < g.K) ? g.A[int64_t(k) * g.M + m] : 0.0;
+    }
+    for (int e = tid; e < GN * GK; e += 256) {
+      const int kk = e % GK, nn = e / GK;
+      const int k = k0 + kk;
+      const int64_t col = bcol[nn];
+      Bs[kk][nn] = (col >= 0 && k < g.K) ? g.B[col + k] * g.kscale[k % g.ksd] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GK; kk++) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) av[i] = As[kk][ty + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; j++) bv[j] = Bs[kk][tx + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) c[i][j] = fma(av[i], bv[j], c[i][j]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int n = n0 + tx + 16 * j;
+    if (n >= g.ncol) continue;
+    double* col = g.C + int64_t(g.cidx[n]) * g.ldc;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int m = m0 + ty + 16 * i;
+      if (m >= g.M) continue;
+      const double v = g.rscale[m % g.rsd] * c[i][j];
+      col[m] = g.accumulate ? col[m] + v : v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+#define KIFMM_USER_KERNELS_MX(X) X(111) X(121) X(311) X(312) X(321) X(322) X(511) X(521)
+
+hipError_t launch_near_mixed(int kt, const NearArgs& a, const KParams<double>& pu, const KParams<double>& pm,
+                             hipStream_t s) {
+  if (a.nchunks <= 0) return hipSuccess;
+  const KParams<float> fu = to_f32(pu), fm = to_f32(pm);
+  switch (kt) {
+#define KIFMM_CASE(K) case K: k_near_mx<K><<<dim3(unsigned(a.nchunks)), dim3(kNearChunk), 0, s>>>(a, fu, fm); break;
+    KIFMM_USER_KERNELS_MX(KIFMM_CASE)
+#undef KIFMM_CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_surface_mixed(int kt, const SurfArgs& a, const KParams<double>& pu, hipStream_t s) {
+  if (a.ntasks <= 0) return hipSuccess;
+  if (a.n_sam > 2 * kSurfMx) return hipErrorInvalidValue;
+  const KParams<float> fu = to_f32(pu);
+  switch (kt) {
+#define KIFMM_CASE(K) case K: k_surface_mx<K><<<dim3(unsigned(a.ntasks)), dim3(kSurfMx), 0, s>>>(a, fu); break;
+    KIFMM_USER_KERNELS_MX(KIFMM_CASE)
+#undef KIFMM_CASE
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_hadamard_mixed(const VlistArgs& v, const float2* dspec, const float2* m2l, float2* espec,
+                                 hipStream_t s) {
+  if (v.ngrp <= 0) return hipSuccess;
+  const float nrm = float(1.0 / double(v.n * v.n * v.n));
+  dim3 grid(unsigned(v.ngrp), unsigned((v.H + 63) / 64));
+  if (v.sd == 1 && v.td == 1) k_hadamard_mx<1, 1><<<grid, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm);
+  else if (v.sd == 4 && v.td == 3) k_hadamard_mx<4, 3><<<grid, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm);
+  else if (v.sd == 3 && v.td == 3) k_hadamard_mx<3, 3><<<grid, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_gemm_fma(const GemmArgs& g, hipStream_t s) {
+  if (g.ncol <= 0) return hipSuccess;
+  dim3 grid(unsigned((g.M + GM - 1) / GM), unsigned((g.ncol + GN - 1) / GN));
+  k_gemm_fma<<<grid, dim3(256), 0, s>>>(g);
+  return hipGetLastError();
+}
+
+}  // namespace kifmm

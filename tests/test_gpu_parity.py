@@ -150,3 +150,32 @@ def test_device_pointers(gpu):
     f.evaluate_device(d, v)
     torch.cuda.synchronize()
     assert np.array_equal(v.cpu().numpy(), host)
+
+
+@pytest.mark.parametrize("kt,np_,coefs,gen,tol", [(111, 6, [1.0, 0.25], "gauss", 3e-7),
+                                                  (111, 4, [1.0, 0.25], "uniform", 3e-7),
+                                                  (321, 4, [1.0, 0.25], "spheres", 3e-6),
+                                                  (511, 6, [1.0, 0.3], "uniform", 1e-6)])
+def test_mixed_precision(kt, np_, coefs, gen, tol):
+    """Mixed path (FP32 near field, S2M/S2L and V-list spectra; FP64 pseudo-inverse
+    applies and final sums): close to the FP64 path and no less accurate
+    against the direct sum beyond that difference."""
+    n = 20000
+    if gen == "spheres":
+        pos, nor = W.spheres(n, 91, nsph=6)
+    else:
+        pos = W.gaussians(n, 92) if gen == "gauss" else W.uniform(n, 93)
+        nor = np.zeros_like(pos)
+    den = W.densities(n, O.src_dof(kt), 94, kind="normal")
+    print("case kt=%d np=%d mixed" % (kt, np_), flush=True)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        f = kifmm.FMM3d(kt, coefs, np_, 32, 30, precision=prec)
+        f.set_points(pos, nor)
+        f.setup()
+        out[prec] = f.evaluate(den)
+        out[prec + "_err"] = f.check(den, out[prec], 2000, seed=7)
+    d = rel(out["mixed"], out["fp64"])
+    print("errors vs direct: fp64 %.3e mixed %.3e" % (out["fp64_err"], out["mixed_err"]), flush=True)
+    assert d < tol
+    assert out["mixed_err"] <= out["fp64_err"] + 2 * d
