@@ -53,6 +53,9 @@ struct GemmArgs {
   const double* B; const int* bidx; int64_t ldb; const double* kscale; int ksd;
   double* C; const int* cidx; int64_t ldc; const double* rscale; int rsd;
   int ncol; int accumulate;
+  // several matrices in one launch: matrix z is A + z*a_zstride and owns the
+  // columns [zoff[z], zoff[z+1]); zoff == nullptr means one matrix, all columns
+  const int* zoff; int nz; int64_t a_zstride; int max_zcol;
 };
 
 struct VlistArgs {

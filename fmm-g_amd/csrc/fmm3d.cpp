@@ -207,11 +207,22 @@ int FMM3d::build_tasks() {
         lb[c].push_back(p); lc[c].push_back(i);
       }
     }
-    if (d >= 3) {
+    if (d >= 3) {  // the 8 child matrices of the level in one launch
       int ks = scale_vec(d - 1 + rl, +1);
-      for (int c = 0; c < 8; c++) {
-        work_.l2l_cols += double(lc[c].size());
-        add_step(dn_steps_, 3 + c, 2, 3, lb[c], lc[c], ks, -1, 1);
+      std::vector<int> b, c;
+      int zoff[9] = {0}, mz = 0;
+      for (int q = 0; q < 8; q++) {
+        b.insert(b.end(), lb[q].begin(), lb[q].end());
+        c.insert(c.end(), lc[q].begin(), lc[q].end());
+        zoff[q + 1] = int(b.size());
+        mz = std::max(mz, int(lb[q].size()));
+      }
+      work_.l2l_cols += double(c.size());
+      if (!b.empty()) {
+        add_step(dn_steps_, 3, 2, 3, b, c, ks, -1, 1);
+        dn_steps_.back().zoff_off = int64_t(lvl_idx_h_.size());
+        dn_steps_.back().max_zcol = mz;
+        lvl_idx_h_.insert(lvl_idx_h_.end(), zoff, zoff + 9);
       }
     }
     work_.dn_cols += double(pb.size());
@@ -435,7 +446,14 @@ int FMM3d::evaluate(const double* d_src_den, double* d_trg_val, hipStream_t s) {
         case 2: g.M = op.dat(DE); g.K = op.dat(DC); g.A = pdn_d_.as<double>(); break;
         default:
           g.M = op.dat(DC); g.K = op.dat(DE);
-          g.A = l2l_d_.as<double>() + size_t(st.mat - 3) * op.dat(DC) * op.dat(DE);
+          g.A = l2l_d_.as<double>();
+      }
+      g.max_zcol = st.ncol;
+      if (st.zoff_off >= 0) {
+        g.zoff = lvl_idx_d_.as<int>() + st.zoff_off;
+        g.nz = 8;
+        g.a_zstride = int64_t(op.dat(DC)) * op.dat(DE);
+        g.max_zcol = st.max_zcol;
       }
       g.B = bufs[st.src]; g.ldb = szs[st.src];
       g.C = bufs[st.dst]; g.ldc = szs[st.dst];

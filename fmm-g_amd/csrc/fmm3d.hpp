@@ -116,6 +116,8 @@ class FMM3d {
     int mat, src, dst, ncol, accumulate;
     int64_t idx_off;  // bidx at idx_off, cidx at idx_off + ncol
     int ks_off, rs_off;
+    int64_t zoff_off = -1;  // 9 column offsets of the 8 L2L matrices, in lvl_idx_h_
+    int max_zcol = 0;
   };
   std::vector<GemmStep> up_steps_, dn_steps_;
   std::vector<int> lvl_idx_h_;

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include "device.hpp"
+#include "gemm.hpp"
 
 namespace kifmm {
 
@@ -235,82 +236,8 @@ __global__ __launch_bounds__(kSurfThreads) void k_surface(SurfArgs a, KParams<do
 }
 
 // ---------------------------------------------------------------------------
-// Indexed batched GEMM: C[:, cidx[j]] = (acc ? C : 0) + rs (.) (A * (ks (.) B[:, bidx[j]]))
-// A is M x K column-major; columns of B/C are contiguous.  64x64 tile per
-// workgroup, 4x4 outputs per lane, K staged through LDS in slabs of 16.
-// ---------------------------------------------------------------------------
-constexpr int GT_M = 64, GT_N = 64, GT_K = 16;
-
-__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
-  __shared__ double As[GT_K][GT_M + 1];
-  __shared__ double Bs[GT_K][GT_N + 1];
-  __shared__ int64_t bcol[GT_N];
-  const int m0 = blockIdx.x * GT_M, n0 = blockIdx.y * GT_N;
-  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-  if (tid < GT_N) {
-    const int j = n0 + tid;
-    bcol[tid] = j < g.ncol ? int64_t(g.bidx[j]) * g.ldb : int64_t(-1);
-  }
-  // Accumulating steps start from the current C entry and add term by term
-  // (the order of dgemv(1.0, A, x, 1.0, y)); overwriting steps start from 0.
-  double c[4][4];
-#pragma unroll
-  for (int j = 0; j < 4; j++) {
-    const int n = n0 + tx + 16 * j;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int m = m0 + ty + 16 * i;
-      c[i][j] = (g.accumulate && n < g.ncol && m < g.M) ? g.C[int64_t(g.cidx[n]) * g.ldc + m] : 0.0;
-    }
-  }
-  __syncthreads();
-  for (int k0 = 0; k0 < g.K; k0 += GT_K) {
-    for (int e = tid; e < GT_M * GT_K; e += 256) {
-      const int mm = e % GT_M, kk = e / GT_M;
-      const int m = m0 + mm, k = k0 + kk;
-      As[kk][mm] = (m < g.M && k < g.K) ? g.A[int64_t(k) * g.M + m] : 0.0;
-    }
-    for (int e = tid; e < GT_N * GT_K; e += 256) {
-      const int kk = e % GT_K, nn = e / GT_K;
-      const int k = k0 + kk;
-      const int64_t col = bcol[nn];
-      Bs[kk][nn] = (col >= 0 && k < g.K) ? g.B[col + k] * g.kscale[k % g.ksd] : 0.0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < GT_K; kk++) {
-      double av[4], bv[4];
-#pragma unroll
-      for (int i = 0; i < 4; i++) av[i] = As[kk][ty + 16 * i];
-#pragma unroll
-      for (int j = 0; j < 4; j++) bv[j] = Bs[kk][tx + 16 * j];
-#pragma unroll
-      for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) c[i][j] += av[i] * bv[j];
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int j = 0; j < 4; j++) {
-    const int n = n0 + tx + 16 * j;
-    if (n >= g.ncol) continue;
-    double* col = g.C + int64_t(g.cidx[n]) * g.ldc;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int m = m0 + ty + 16 * i;
-      if (m >= g.M) continue;
-      col[m] = g.accumulate ? c[i][j] : g.rscale[m % g.rsd] * c[i][j];
-    }
-  }
-}
-
-hipError_t launch_gemm(const GemmArgs& g, hipStream_t s) {
-  if (g.ncol <= 0) return hipSuccess;
-  dim3 grid(unsigned((g.M + GT_M - 1) / GT_M), unsigned((g.ncol + GT_N - 1) / GT_N));
-  k_gemm<<<grid, dim3(256), 0, s>>>(g);
-  return hipGetLastError();
-}
+// Level-batched translations and pseudo-inverse applies: see gemm.hpp.
+hipError_t launch_gemm(const GemmArgs& g, hipStream_t s) { return launch_gemm_t<false>(g, s); }
 
 // ---------------------------------------------------------------------------
 // V list.  Forward: UE samples (scaled 2^(l*deg)) scattered onto the (2np)^3
@@ -337,13 +264,13 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_fwd(VlistArgs v, C2* dspec)
   __shared__ double xb[kMaxNp * kMaxNp * kMaxNp];
   __shared__ double2 t1[kMaxNp * kMaxNp * kMaxH];
   __shared__ double2 t2[kMaxNp * kMaxN * kMaxH];
-  __shared__ double2 w[kMaxN];
+  __shared__ double2 w[kMaxN * kMaxN];  // w[x*n+k] = exp(-2 pi i (x*k mod n)/n)
   const int slot = blockIdx.x;
   const int node = v.src_node[slot];
   const int np = v.np, n = v.n, h = n / 2 + 1, o = np / 2, H = v.H;
   const int tid = threadIdx.x;
   const int nsam = np * np * np - (np - 2) * (np - 2) * (np - 2);
-  if (tid < n) w[tid] = v.twiddle[tid];
+  for (int e = tid; e < n * n; e += kFftThreads) w[e] = v.twiddle[(e / n) * (e % n) % n];
   const int l = v.depth[node] + v.root_level;
   for (int s = 0; s < v.sd; s++) {
     const double scale = ldexp(1.0, int(l * v.deg[s]));
@@ -358,7 +285,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_fwd(VlistArgs v, C2* dspec)
       const int kc = e % h, ab = e / h;
       double2 acc = make_double2(0, 0);
       for (int c = 0; c < np; c++) {
-        const double2 tw = w[((c + o) * kc) % n];
+        const double2 tw = w[(c + o) * n + kc];
         const double xv = xb[ab * np + c];
         acc.x += xv * tw.x; acc.y += xv * tw.y;
       }
@@ -369,7 +296,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_fwd(VlistArgs v, C2* dspec)
     for (int e = tid; e < np * n * h; e += kFftThreads) {
       const int kc = e % h, kb = (e / h) % n, a = e / (h * n);
       double2 acc = make_double2(0, 0);
-      for (int b = 0; b < np; b++) acc = cadd(acc, cmul(t1[(a * np + b) * h + kc], w[((b + o) * kb) % n]));
+      for (int b = 0; b < np; b++) acc = cadd(acc, cmul(t1[(a * np + b) * h + kc], w[(b + o) * n + kb]));
       t2[e] = acc;
     }
     __syncthreads();
@@ -378,7 +305,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_fwd(VlistArgs v, C2* dspec)
     for (int e = tid; e < H; e += kFftThreads) {
       const int kc = e % h, kb = (e / h) % n, ka = e / (h * n);
       double2 acc = make_double2(0, 0);
-      for (int a = 0; a < np; a++) acc = cadd(acc, cmul(t2[(a * n + kb) * h + kc], w[((a + o) * ka) % n]));
+      for (int a = 0; a < np; a++) acc = cadd(acc, cmul(t2[(a * n + kb) * h + kc], w[(a + o) * n + ka]));
       store_c(out + e, acc);
     }
   }
@@ -444,13 +371,16 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_inv(VlistArgs v, const C2* 
   __shared__ double2 xs[kMaxN * kMaxN * kMaxH];
   __shared__ double2 y1[kMaxNp * kMaxN * kMaxH];
   __shared__ double2 y2[kMaxNp * kMaxNp * kMaxH];
-  __shared__ double2 w[kMaxN];
+  __shared__ double2 w[kMaxN * kMaxN];  // w[x*n+k] = exp(+2 pi i (x*k mod n)/n)
   const int slot = blockIdx.x;
   const int node = v.trg_node[slot];
   const int np = v.np, n = v.n, h = n / 2 + 1, o = np / 2, H = v.H;
   const int tid = threadIdx.x;
   const int nsam = np * np * np - (np - 2) * (np - 2) * (np - 2);
-  if (tid < n) w[tid] = make_double2(v.twiddle[tid].x, -v.twiddle[tid].y);  // e^{+2 pi i m/n}
+  for (int e = tid; e < n * n; e += kFftThreads) {
+    const double2 t = v.twiddle[(e / n) * (e % n) % n];
+    w[e] = make_double2(t.x, -t.y);
+  }
   for (int i = 0; i < v.td; i++) {
     __syncthreads();
     const C2* in = espec + (int64_t(slot) * v.td + i) * H;
@@ -460,7 +390,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_inv(VlistArgs v, const C2* 
     for (int e = tid; e < np * n * h; e += kFftThreads) {
       const int kc = e % h, kb = (e / h) % n, a = e / (h * n);
       double2 acc = make_double2(0, 0);
-      for (int ka = 0; ka < n; ka++) acc = cadd(acc, cmul(xs[(ka * n + kb) * h + kc], w[((a + o) * ka) % n]));
+      for (int ka = 0; ka < n; ka++) acc = cadd(acc, cmul(xs[(ka * n + kb) * h + kc], w[(a + o) * n + ka]));
       y1[e] = acc;
     }
     __syncthreads();
@@ -468,7 +398,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_inv(VlistArgs v, const C2* 
     for (int e = tid; e < np * np * h; e += kFftThreads) {
       const int kc = e % h, b = (e / h) % np, a = e / (h * np);
       double2 acc = make_double2(0, 0);
-      for (int kb = 0; kb < n; kb++) acc = cadd(acc, cmul(y1[(a * n + kb) * h + kc], w[((b + o) * kb) % n]));
+      for (int kb = 0; kb < n; kb++) acc = cadd(acc, cmul(y1[(a * n + kb) * h + kc], w[(b + o) * n + kb]));
       y2[e] = acc;
     }
     __syncthreads();
@@ -481,7 +411,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_inv(VlistArgs v, const C2* 
       const int cg = c + o;
       double val = yy[0].x + yy[n / 2].x * ((cg & 1) ? -1.0 : 1.0);
       for (int kc = 1; kc < n / 2; kc++) {
-        const double2 tw = w[(cg * kc) % n];
+        const double2 tw = w[cg * n + kc];
         val += 2.0 * (yy[kc].x * tw.x - yy[kc].y * tw.y);
       }
       dc[q * v.td + i] += val;

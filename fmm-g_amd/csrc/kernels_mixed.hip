@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include "device.hpp"
+#include "gemm.hpp"
 
 namespace kifmm {
 
@@ -280,63 +281,68 @@ __global__ __launch_bounds__(64) void k_hadamard_mx(VlistArgs v, const float2* _
 }
 
 // ---------------------------------------------------------------------------
-// Indexed batched FP64 GEMM with FMA (same contract as k_gemm).
-constexpr int GM = 64, GN = 64, GK = 16;
+// Scalar-kernel Hadamard (sd = td = 1): the FP32 translation spectra of a
+// 16-frequency slice for all 343 offset slots sit in LDS (47 KB), so a spectrum
+// value is read from LDS instead of L2 for every pair.  Lane = (frequency,
+// sibling group): 16 frequencies x 4 groups per wave; each lane accumulates the
+// 8 target children of its group, loads each halo source spectrum once, and
+// runs 4 sources ahead to keep loads in flight.
+constexpr int kHxF = 16, kHxRow = kHxF + 1, kHxGroupsPerWave = 64 / kHxF;
+constexpr int kHxWaves = 4, kHxBatch = 8;
 
-__global__ __launch_bounds__(256) void k_gemm_fma(GemmArgs g) {
-  __shared__ double As[GK][GM + 1];
-  __shared__ double Bs[GK][GN + 1];
-  __shared__ int64_t bcol[GN];
-  const int m0 = blockIdx.x * GM, n0 = blockIdx.y * GN;
-  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-  if (tid < GN) {
-    const int j = n0 + tid;
-    bcol[tid] = j < g.ncol ? int64_t(g.bidx[j]) * g.ldb : int64_t(-1);
+__global__ __launch_bounds__(64 * kHxWaves) void k_hadamard_mx_s(VlistArgs v, const float2* __restrict__ dspec,
+                                                              const float2* __restrict__ m2l,
+                                                              float2* __restrict__ espec, float nrm) {
+  __shared__ float2 Ts[343 * kHxRow];
+  const int H = v.H;
+  const int f0 = blockIdx.x * kHxF;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int fl = lane % kHxF, gq = lane / kHxF;
+  const int f = f0 + fl;
+  const bool fok = f < H;
+  const int fs = fok ? f : 0;
+  for (int e = tid; e < 343 * kHxF; e += 64 * kHxWaves) {
+    const int sl = e / kHxF, ff = f0 + e % kHxF;
+    Ts[sl * kHxRow + e % kHxF] = ff < H ? m2l[int64_t(sl) * H + ff] : make_float2(0.f, 0.f);
   }
-  double c[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; i++)
-#pragma unroll
-    for (int j = 0; j < 4; j++) c[i][j] = 0.0;
   __syncthreads();
-  for (int k0 = 0; k0 < g.K; k0 += GK) {
-    for (int e = tid; e < GM * GK; e += 256) {
-      const int mm = e % GM, kk = e / GM;
-      const int m = m0 + mm, k = k0 + kk;
-      As[kk][mm] = (m < g.M && k < g.K) ? g.A[int64_t(k) * g.M + m] : 0.0;
+  constexpr int per_pass = kHxWaves * kHxGroupsPerWave;
+  const int gbase = blockIdx.y * kHxBatch * per_pass;
+  for (int pass = 0; pass < kHxBatch; pass++) {
+    const int g = gbase + pass * per_pass + wave * kHxGroupsPerWave + gq;
+    const bool gok = g < v.ngrp;
+    const int cnt = gok ? v.pcnt[g] : 0;
+    const int* ps = v.psrc + int64_t(gok ? g : 0) * kPairStride;
+    const int* pr = v.pslot + int64_t(gok ? g : 0) * kPairStride;
+    float2 acc[8];
+#pragma unroll
+    for (int t = 0; t < 8; t++) acc[t] = make_float2(0.f, 0.f);
+    for (int k = 0; k < cnt; k += 4) {  // cnt is a multiple of 8
+      int sr[4], wd[4];
+      float2 d[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) { sr[u] = ps[k + u]; wd[u] = pr[k + u]; }
+#pragma unroll
+      for (int u = 0; u < 4; u++) d[u] = dspec[int64_t(sr[u] < 0 ? 0 : sr[u]) * H + fs];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        if (sr[u] < 0) continue;
+        const int rs = (wd[u] >> 8) & 0x3ff, tm = wd[u] >> 20;
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+          if (!((tm >> t) & 1)) continue;
+          const float2 tt = Ts[(rs - ((t >> 2) * 49 + ((t >> 1) & 1) * 7 + (t & 1))) * kHxRow + fl];
+          acc[t].x = fmaf(tt.x, d[u].x, fmaf(-tt.y, d[u].y, acc[t].x));
+          acc[t].y = fmaf(tt.x, d[u].y, fmaf(tt.y, d[u].x, acc[t].y));
+        }
+      }
     }
-    for (int e = tid; e < GN * GK; e += 256) {
-      const int kk = e % GK, nn = e / GK;
-      const int k = k0 + kk;
-      const int64_t col = bcol[nn];
-      Bs[kk][nn] = (col >= 0 && k < g.K) ? g.B[col + k] * g.kscale[k % g.ksd] : 0.0;
-    }
-    __syncthreads();
+    if (gok && fok) {
 #pragma unroll
-    for (int kk = 0; kk < GK; kk++) {
-      double av[4], bv[4];
-#pragma unroll
-      for (int i = 0; i < 4; i++) av[i] = As[kk][ty + 16 * i];
-#pragma unroll
-      for (int j = 0; j < 4; j++) bv[j] = Bs[kk][tx + 16 * j];
-#pragma unroll
-      for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) c[i][j] = fma(av[i], bv[j], c[i][j]);
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int j = 0; j < 4; j++) {
-    const int n = n0 + tx + 16 * j;
-    if (n >= g.ncol) continue;
-    double* col = g.C + int64_t(g.cidx[n]) * g.ldc;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int m = m0 + ty + 16 * i;
-      if (m >= g.M) continue;
-      const double v = g.rscale[m % g.rsd] * c[i][j];
-      col[m] = g.accumulate ? col[m] + v : v;
+      for (int t = 0; t < 8; t++) {
+        const int tg = v.tgt[g * 8 + t];
+        if (tg >= 0) espec[int64_t(tg) * H + f] = make_float2(acc[t].x * nrm, acc[t].y * nrm);
+      }
     }
   }
 }
@@ -375,18 +381,17 @@ hipError_t launch_hadamard_mixed(const VlistArgs& v, const float2* dspec, const 
   if (v.ngrp <= 0) return hipSuccess;
   const float nrm = float(1.0 / double(v.n * v.n * v.n));
   dim3 grid(unsigned(v.ngrp), unsigned((v.H + 63) / 64));
-  if (v.sd == 1 && v.td == 1) k_hadamard_mx<1, 1><<<grid, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm);
+  if (v.sd == 1 && v.td == 1) {
+    constexpr int per_wg = kHxBatch * kHxWaves * kHxGroupsPerWave;
+    k_hadamard_mx_s<<<dim3(unsigned((v.H + kHxF - 1) / kHxF), unsigned((v.ngrp + per_wg - 1) / per_wg)),
+                      dim3(64 * kHxWaves), 0, s>>>(v, dspec, m2l, espec, nrm);
+  }
   else if (v.sd == 4 && v.td == 3) k_hadamard_mx<4, 3><<<grid, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm);
   else if (v.sd == 3 && v.td == 3) k_hadamard_mx<3, 3><<<grid, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
-hipError_t launch_gemm_fma(const GemmArgs& g, hipStream_t s) {
-  if (g.ncol <= 0) return hipSuccess;
-  dim3 grid(unsigned((g.M + GM - 1) / GM), unsigned((g.ncol + GN - 1) / GN));
-  k_gemm_fma<<<grid, dim3(256), 0, s>>>(g);
-  return hipGetLastError();
-}
+hipError_t launch_gemm_fma(const GemmArgs& g, hipStream_t s) { return launch_gemm_t<true>(g, s); }
 
 }  // namespace kifmm

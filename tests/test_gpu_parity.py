@@ -154,7 +154,7 @@ def test_device_pointers(gpu):
 
 @pytest.mark.parametrize("kt,np_,coefs,gen,tol", [(111, 6, [1.0, 0.25], "gauss", 3e-7),
                                                   (111, 4, [1.0, 0.25], "uniform", 3e-7),
-                                                  (321, 4, [1.0, 0.25], "spheres", 3e-6),
+                                                  (321, 4, [1.0, 0.25], "spheres", 3e-5),
                                                   (511, 6, [1.0, 0.3], "uniform", 1e-6)])
 def test_mixed_precision(kt, np_, coefs, gen, tol):
     """Mixed path (FP32 near field, S2M/S2L and V-list spectra; FP64 pseudo-inverse
