@@ -269,6 +269,21 @@ int FMM3d::build_tasks() {
   }
   // per group: halo slots ordered by node index (the V lists' order)
   const int ngrp = int(grp.size());
+  // The halo-sweep Hadamard relies on V(tau) = {occupied halo slots not adjacent
+  // to tau} for every target child tau of a group; check it on the lists.
+  for (int g = 0; g < ngrp; g++)
+    for (int tau = 0; tau < 8; tau++) {
+      if (vtgt_h_[size_t(g) * 8 + tau] < 0) continue;
+      const int tc[3] = {tau >> 2, (tau >> 1) & 1, tau & 1};
+      for (int r = 0; r < 216; r++) {
+        const int rc[3] = {r / 36 - 2, (r / 6) % 6 - 2, r % 6 - 2};
+        bool adj = true;
+        for (int d = 0; d < 3; d++) adj = adj && std::abs(rc[d] - tc[d]) <= 1;
+        const bool in_v = (vmask_h_[size_t(g) * 32 + tau * 4 + (r >> 6)] >> (r & 63)) & 1;
+        if (in_v != (!adj && vhalo_h_[size_t(g) * 216 + r] >= 0))
+          return fail(-6, "V list differs from the non-adjacent halo of its sibling group");
+      }
+    }
   vhord_h_.assign(size_t(ngrp) * 216, -1);
   for (auto& pg : grp) {
     const int g = pg.second;

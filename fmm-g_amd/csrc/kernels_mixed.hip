@@ -281,68 +281,80 @@ __global__ __launch_bounds__(64) void k_hadamard_mx(VlistArgs v, const float2* _
 }
 
 // ---------------------------------------------------------------------------
-// Scalar-kernel Hadamard (sd = td = 1): the FP32 translation spectra of a
-// 16-frequency slice for all 343 offset slots sit in LDS (47 KB), so a spectrum
-// value is read from LDS instead of L2 for every pair.  Lane = (frequency,
-// sibling group): 16 frequencies x 4 groups per wave; each lane accumulates the
-// 8 target children of its group, loads each halo source spectrum once, and
-// runs 4 sources ahead to keep loads in flight.
-constexpr int kHxF = 16, kHxRow = kHxF + 1, kHxGroupsPerWave = 64 / kHxF;
-constexpr int kHxWaves = 4, kHxBatch = 8;
+// Scalar-kernel Hadamard (sd = td = 1) over tiles of sibling groups.  The V list
+// of a target child tau of parent P is every source child c of a colleague Q of
+// P (Q = P + q, q in [-1,1]^3, q != 0) that is not adjacent to tau; the
+// translation slot is 2q + (c - tau), and the 27 slots with |offset| <= 1 hold
+// zeros, so the whole halo can be swept with no per-pair lists.  Lane =
+// frequency (64 per wave), so every index, halo lookup and branch is uniform
+// across the wave and runs on the scalar unit; for one colleague offset q the
+// 27 translation spectra 2q + [-1,1]^3 sit in registers and serve all kTileG
+// groups of the tile.  Absent halo sources (-1) are skipped; targets outside
+// the V-list target set are computed and dropped.  Frequency chunks are the
+// slow grid dimension and each XCD (workgroup id mod 8) sweeps a contiguous run
+// of tiles, so the spectra of neighbouring groups are re-read from its L2.
+constexpr int kTileG = 8;
 
-__global__ __launch_bounds__(64 * kHxWaves) void k_hadamard_mx_s(VlistArgs v, const float2* __restrict__ dspec,
-                                                              const float2* __restrict__ m2l,
-                                                              float2* __restrict__ espec, float nrm) {
-  __shared__ float2 Ts[343 * kHxRow];
+__global__ __launch_bounds__(64) void k_m2l_tile(VlistArgs v, const float2* __restrict__ dspec,
+                                                 const float2* __restrict__ m2l, float2* __restrict__ espec,
+                                                 float nrm, int ntile, int per_xcd) {
+  const int bx = blockIdx.x;
+  const int tile = (bx & 7) * per_xcd + (bx >> 3);
+  if (tile >= ntile) return;
   const int H = v.H;
-  const int f0 = blockIdx.x * kHxF;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int fl = lane % kHxF, gq = lane / kHxF;
-  const int f = f0 + fl;
+  const int f = blockIdx.y * 64 + threadIdx.x;
   const bool fok = f < H;
-  const int fs = fok ? f : 0;
-  for (int e = tid; e < 343 * kHxF; e += 64 * kHxWaves) {
-    const int sl = e / kHxF, ff = f0 + e % kHxF;
-    Ts[sl * kHxRow + e % kHxF] = ff < H ? m2l[int64_t(sl) * H + ff] : make_float2(0.f, 0.f);
-  }
-  __syncthreads();
-  constexpr int per_pass = kHxWaves * kHxGroupsPerWave;
-  const int gbase = blockIdx.y * kHxBatch * per_pass;
-  for (int pass = 0; pass < kHxBatch; pass++) {
-    const int g = gbase + pass * per_pass + wave * kHxGroupsPerWave + gq;
-    const bool gok = g < v.ngrp;
-    const int cnt = gok ? v.pcnt[g] : 0;
-    const int* ps = v.psrc + int64_t(gok ? g : 0) * kPairStride;
-    const int* pr = v.pslot + int64_t(gok ? g : 0) * kPairStride;
-    float2 acc[8];
+  const int fs = fok ? f : H - 1;
+  const int g0 = tile * kTileG;
+  const int ng = min(kTileG, v.ngrp - g0);
+  float2 acc[kTileG][8];
 #pragma unroll
-    for (int t = 0; t < 8; t++) acc[t] = make_float2(0.f, 0.f);
-    for (int k = 0; k < cnt; k += 4) {  // cnt is a multiple of 8
-      int sr[4], wd[4];
-      float2 d[4];
+  for (int gi = 0; gi < kTileG; gi++)
 #pragma unroll
-      for (int u = 0; u < 4; u++) { sr[u] = ps[k + u]; wd[u] = pr[k + u]; }
+    for (int t = 0; t < 8; t++) acc[gi][t] = make_float2(0.f, 0.f);
+  for (int q = 0; q < 27; q++) {
+    if (q == 13) continue;  // the parent itself: siblings are adjacent
+    const int qx = q / 9 - 1, qy = (q / 3) % 3 - 1, qz = q % 3 - 1;
+    const int sbase = (2 * qx + 3) * 49 + (2 * qy + 3) * 7 + (2 * qz + 3);
+    float2 T[27];
 #pragma unroll
-      for (int u = 0; u < 4; u++) d[u] = dspec[int64_t(sr[u] < 0 ? 0 : sr[u]) * H + fs];
+    for (int d = 0; d < 27; d++) {
+      const int sl = sbase + (d / 9 - 1) * 49 + ((d / 3) % 3 - 1) * 7 + (d % 3 - 1);
+      T[d] = m2l[int64_t(sl) * H + fs];
+    }
+    const int hbase = (2 * qx + 2) * 36 + (2 * qy + 2) * 6 + (2 * qz + 2);
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        if (sr[u] < 0) continue;
-        const int rs = (wd[u] >> 8) & 0x3ff, tm = wd[u] >> 20;
+    for (int gi = 0; gi < kTileG; gi++) {
+      if (gi < ng) {
+        const int* hal = v.halo + int64_t(g0 + gi) * 216 + hbase;
+        int si[8];
 #pragma unroll
-        for (int t = 0; t < 8; t++) {
-          if (!((tm >> t) & 1)) continue;
-          const float2 tt = Ts[(rs - ((t >> 2) * 49 + ((t >> 1) & 1) * 7 + (t & 1))) * kHxRow + fl];
-          acc[t].x = fmaf(tt.x, d[u].x, fmaf(-tt.y, d[u].y, acc[t].x));
-          acc[t].y = fmaf(tt.x, d[u].y, fmaf(tt.y, d[u].x, acc[t].y));
+        for (int c = 0; c < 8; c++) si[c] = hal[(c >> 2) * 36 + ((c >> 1) & 1) * 6 + (c & 1)];
+        float2 x[8];
+#pragma unroll
+        for (int c = 0; c < 8; c++) x[c] = dspec[int64_t(si[c] < 0 ? 0 : si[c]) * H + fs];
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+          if (si[c] < 0) continue;
+#pragma unroll
+          for (int t = 0; t < 8; t++) {
+            const int d = ((c >> 2) - (t >> 2) + 1) * 9 + (((c >> 1) & 1) - ((t >> 1) & 1) + 1) * 3 +
+                          ((c & 1) - (t & 1) + 1);
+            acc[gi][t].x = fmaf(T[d].x, x[c].x, fmaf(-T[d].y, x[c].y, acc[gi][t].x));
+            acc[gi][t].y = fmaf(T[d].x, x[c].y, fmaf(T[d].y, x[c].x, acc[gi][t].y));
+          }
         }
       }
     }
-    if (gok && fok) {
+  }
+  if (!fok) return;
 #pragma unroll
-      for (int t = 0; t < 8; t++) {
-        const int tg = v.tgt[g * 8 + t];
-        if (tg >= 0) espec[int64_t(tg) * H + f] = make_float2(acc[t].x * nrm, acc[t].y * nrm);
-      }
+  for (int gi = 0; gi < kTileG; gi++) {
+    if (gi >= ng) break;
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+      const int tg = v.tgt[(g0 + gi) * 8 + t];
+      if (tg >= 0) espec[int64_t(tg) * H + f] = make_float2(acc[gi][t].x * nrm, acc[gi][t].y * nrm);
     }
   }
 }
@@ -382,9 +394,10 @@ hipError_t launch_hadamard_mixed(const VlistArgs& v, const float2* dspec, const 
   const float nrm = float(1.0 / double(v.n * v.n * v.n));
   dim3 grid(unsigned(v.ngrp), unsigned((v.H + 63) / 64));
   if (v.sd == 1 && v.td == 1) {
-    constexpr int per_wg = kHxBatch * kHxWaves * kHxGroupsPerWave;
-    k_hadamard_mx_s<<<dim3(unsigned((v.H + kHxF - 1) / kHxF), unsigned((v.ngrp + per_wg - 1) / per_wg)),
-                      dim3(64 * kHxWaves), 0, s>>>(v, dspec, m2l, espec, nrm);
+    const int ntile = (v.ngrp + kTileG - 1) / kTileG;
+    const int per_xcd = (ntile + 7) / 8;
+    k_m2l_tile<<<dim3(unsigned(8 * per_xcd), unsigned((v.H + 63) / 64)), dim3(64), 0, s>>>(
+        v, dspec, m2l, espec, nrm, ntile, per_xcd);
   }
   else if (v.sd == 4 && v.td == 3) k_hadamard_mx<4, 3><<<grid, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm);
   else if (v.sd == 3 && v.td == 3) k_hadamard_mx<3, 3><<<grid, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm);
