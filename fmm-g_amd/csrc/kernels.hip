@@ -242,11 +242,14 @@ hipError_t launch_gemm(const GemmArgs& g, hipStream_t s) { return launch_gemm_t<
 // ---------------------------------------------------------------------------
 // V list.  Forward: UE samples (scaled 2^(l*deg)) scattered onto the (2np)^3
 // grid and transformed (r2c, FFTW layout).  Only the np^3 block at offset np/2
-// is non-zero, so the three DFT passes run over that block.
+// is non-zero, so the three DFT passes run over that block.  One wave per
+// node; each lane transforms whole lines, and since the twiddle exponent of
+// input x and output k is ((x + np/2) k) mod n with x and k unrolled at compile
+// time (NP is a template parameter), the n twiddles sit in registers.  Every
+// output is summed over its inputs in ascending order, as the separable DFT of
+// the reference layout does.
 // ---------------------------------------------------------------------------
-constexpr int kFftThreads = 256;
 constexpr int kMaxNp = 8;
-constexpr int kMaxN = 2 * kMaxNp, kMaxH = kMaxN / 2 + 1;
 
 __device__ inline double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -259,54 +262,72 @@ __device__ inline double2 load_c(const double2* p) { return *p; }
 __device__ inline double2 load_c(const float2* p) { const float2 v = *p; return make_double2(v.x, v.y); }
 
 // C2 = storage type of the spectra (double2, or float2 for the mixed path).
-template <typename C2>
-__global__ __launch_bounds__(kFftThreads) void k_fft_fwd(VlistArgs v, C2* dspec) {
-  __shared__ double xb[kMaxNp * kMaxNp * kMaxNp];
-  __shared__ double2 t1[kMaxNp * kMaxNp * kMaxH];
-  __shared__ double2 t2[kMaxNp * kMaxN * kMaxH];
-  __shared__ double2 w[kMaxN * kMaxN];  // w[x*n+k] = exp(-2 pi i (x*k mod n)/n)
+template <int NP, typename C2>
+__global__ __launch_bounds__(64) void k_fft_fwd(VlistArgs v, C2* dspec) {
+  constexpr int N = 2 * NP, HH = NP + 1, O = NP / 2;
+  constexpr int NSAM = NP * NP * NP - (NP - 2) * (NP - 2) * (NP - 2);
+  __shared__ double xb[NP * NP * NP];
+  __shared__ double2 t1[NP * NP * HH];
+  __shared__ double2 t2[NP * N * HH];
   const int slot = blockIdx.x;
   const int node = v.src_node[slot];
-  const int np = v.np, n = v.n, h = n / 2 + 1, o = np / 2, H = v.H;
-  const int tid = threadIdx.x;
-  const int nsam = np * np * np - (np - 2) * (np - 2) * (np - 2);
-  for (int e = tid; e < n * n; e += kFftThreads) w[e] = v.twiddle[(e / n) * (e % n) % n];
+  const int lane = threadIdx.x;
+  double2 w[N];  // exp(-2 pi i m / n)
+#pragma unroll
+  for (int m = 0; m < N; m++) w[m] = v.twiddle[m];
   const int l = v.depth[node] + v.root_level;
   for (int s = 0; s < v.sd; s++) {
     const double scale = ldexp(1.0, int(l * v.deg[s]));
     __syncthreads();
-    for (int e = tid; e < np * np * np; e += kFftThreads) xb[e] = 0.0;
+    for (int e = lane; e < NP * NP * NP; e += 64) xb[e] = 0.0;
     __syncthreads();
-    for (int q = tid; q < nsam; q += kFftThreads)
-      xb[v.shell[q]] = v.UE[int64_t(node) * v.sz_ue + q * v.sd + s] * scale;
+    for (int q = lane; q < NSAM; q += 64) xb[v.shell[q]] = v.UE[int64_t(node) * v.sz_ue + q * v.sd + s] * scale;
     __syncthreads();
-    // along x (c): real -> half spectrum
-    for (int e = tid; e < np * np * h; e += kFftThreads) {
-      const int kc = e % h, ab = e / h;
-      double2 acc = make_double2(0, 0);
-      for (int c = 0; c < np; c++) {
-        const double2 tw = w[(c + o) * n + kc];
-        const double xv = xb[ab * np + c];
-        acc.x += xv * tw.x; acc.y += xv * tw.y;
+    // along x (c): real -> half spectrum; line (a, b)
+    for (int ab = lane; ab < NP * NP; ab += 64) {
+      double xv[NP];
+#pragma unroll
+      for (int c = 0; c < NP; c++) xv[c] = xb[ab * NP + c];
+#pragma unroll
+      for (int kc = 0; kc < HH; kc++) {
+        double2 acc = make_double2(0, 0);
+#pragma unroll
+        for (int c = 0; c < NP; c++) {
+          const double2 tw = w[((c + O) * kc) % N];
+          acc.x += xv[c] * tw.x; acc.y += xv[c] * tw.y;
+        }
+        t1[ab * HH + kc] = acc;
       }
-      t1[e] = acc;
     }
     __syncthreads();
-    // along y (b)
-    for (int e = tid; e < np * n * h; e += kFftThreads) {
-      const int kc = e % h, kb = (e / h) % n, a = e / (h * n);
-      double2 acc = make_double2(0, 0);
-      for (int b = 0; b < np; b++) acc = cadd(acc, cmul(t1[(a * np + b) * h + kc], w[(b + o) * n + kb]));
-      t2[e] = acc;
+    // along y (b); line (a, kc)
+    for (int e = lane; e < NP * HH; e += 64) {
+      const int a = e / HH, kc = e % HH;
+      double2 in[NP];
+#pragma unroll
+      for (int b = 0; b < NP; b++) in[b] = t1[(a * NP + b) * HH + kc];
+#pragma unroll
+      for (int kb = 0; kb < N; kb++) {
+        double2 acc = make_double2(0, 0);
+#pragma unroll
+        for (int b = 0; b < NP; b++) acc = cadd(acc, cmul(in[b], w[((b + O) * kb) % N]));
+        t2[(a * N + kb) * HH + kc] = acc;
+      }
     }
     __syncthreads();
-    // along z (a) -> global, frequency index (ka*n+kb)*h+kc
-    C2* out = dspec + (int64_t(slot) * v.sd + s) * H;
-    for (int e = tid; e < H; e += kFftThreads) {
-      const int kc = e % h, kb = (e / h) % n, ka = e / (h * n);
-      double2 acc = make_double2(0, 0);
-      for (int a = 0; a < np; a++) acc = cadd(acc, cmul(t2[(a * n + kb) * h + kc], w[(a + o) * n + ka]));
-      store_c(out + e, acc);
+    // along z (a) -> global, frequency index (ka*n+kb)*h+kc; line (kb, kc)
+    C2* out = dspec + (int64_t(slot) * v.sd + s) * (N * N * HH);
+    for (int e = lane; e < N * HH; e += 64) {
+      double2 in[NP];
+#pragma unroll
+      for (int a = 0; a < NP; a++) in[a] = t2[a * N * HH + e];
+#pragma unroll
+      for (int ka = 0; ka < N; ka++) {
+        double2 acc = make_double2(0, 0);
+#pragma unroll
+        for (int a = 0; a < NP; a++) acc = cadd(acc, cmul(in[a], w[((a + O) * ka) % N]));
+        store_c(out + ka * N * HH + e, acc);
+      }
     }
   }
 }
@@ -365,65 +386,113 @@ __global__ __launch_bounds__(64) void k_hadamard(VlistArgs v, double nrm) {
 }
 
 // Inverse: unnormalised c2r of each target spectrum, evaluated only on the
-// surface samples, added into DC (effVal2PlnVal + regVal2SamVal).
-template <typename C2>
-__global__ __launch_bounds__(kFftThreads) void k_fft_inv(VlistArgs v, const C2* espec) {
-  __shared__ double2 xs[kMaxN * kMaxN * kMaxH];
-  __shared__ double2 y1[kMaxNp * kMaxN * kMaxH];
-  __shared__ double2 y2[kMaxNp * kMaxNp * kMaxH];
-  __shared__ double2 w[kMaxN * kMaxN];  // w[x*n+k] = exp(+2 pi i (x*k mod n)/n)
+// surface samples, added into DC (effVal2PlnVal + regVal2SamVal).  Same line
+// scheme as the forward transform; the first pass reads the spectrum straight
+// from global memory (lines are contiguous across lanes).
+template <int NP, typename C2>
+__global__ __launch_bounds__(64) void k_fft_inv(VlistArgs v, const C2* espec) {
+  constexpr int N = 2 * NP, HH = NP + 1, O = NP / 2;
+  constexpr int NSAM = NP * NP * NP - (NP - 2) * (NP - 2) * (NP - 2);
+  __shared__ double2 y1[NP * N * HH];
+  __shared__ double2 y2[NP * NP * HH];
+  __shared__ int qof[NP * NP * NP];  // block index -> surface sample, -1 inside
   const int slot = blockIdx.x;
   const int node = v.trg_node[slot];
-  const int np = v.np, n = v.n, h = n / 2 + 1, o = np / 2, H = v.H;
-  const int tid = threadIdx.x;
-  const int nsam = np * np * np - (np - 2) * (np - 2) * (np - 2);
-  for (int e = tid; e < n * n; e += kFftThreads) {
-    const double2 t = v.twiddle[(e / n) * (e % n) % n];
-    w[e] = make_double2(t.x, -t.y);
-  }
+  const int lane = threadIdx.x;
+  double2 w[N];  // exp(+2 pi i m / n)
+#pragma unroll
+  for (int m = 0; m < N; m++) { const double2 t = v.twiddle[m]; w[m] = make_double2(t.x, -t.y); }
+  for (int e = lane; e < NP * NP * NP; e += 64) qof[e] = -1;
+  __syncthreads();
+  for (int q = lane; q < NSAM; q += 64) qof[v.shell[q]] = q;
+  double* dc = v.DC + int64_t(node) * v.sz_dc;
   for (int i = 0; i < v.td; i++) {
+    const C2* in = espec + (int64_t(slot) * v.td + i) * (N * N * HH);
     __syncthreads();
-    const C2* in = espec + (int64_t(slot) * v.td + i) * H;
-    for (int e = tid; e < H; e += kFftThreads) xs[e] = load_c(in + e);
-    __syncthreads();
-    // along z (a), only block rows a' in [0,np)
-    for (int e = tid; e < np * n * h; e += kFftThreads) {
-      const int kc = e % h, kb = (e / h) % n, a = e / (h * n);
-      double2 acc = make_double2(0, 0);
-      for (int ka = 0; ka < n; ka++) acc = cadd(acc, cmul(xs[(ka * n + kb) * h + kc], w[(a + o) * n + ka]));
-      y1[e] = acc;
-    }
-    __syncthreads();
-    // along y (b), only b' in [0,np)
-    for (int e = tid; e < np * np * h; e += kFftThreads) {
-      const int kc = e % h, b = (e / h) % np, a = e / (h * np);
-      double2 acc = make_double2(0, 0);
-      for (int kb = 0; kb < n; kb++) acc = cadd(acc, cmul(y1[(a * n + kb) * h + kc], w[(b + o) * n + kb]));
-      y2[e] = acc;
-    }
-    __syncthreads();
-    // along x (c), c2r at the surface samples
-    double* dc = v.DC + int64_t(node) * v.sz_dc;
-    for (int q = tid; q < nsam; q += kFftThreads) {
-      const int blk = v.shell[q];
-      const int c = blk % np, ab = blk / np;
-      const double2* yy = y2 + ab * h;
-      const int cg = c + o;
-      double val = yy[0].x + yy[n / 2].x * ((cg & 1) ? -1.0 : 1.0);
-      for (int kc = 1; kc < n / 2; kc++) {
-        const double2 tw = w[cg * n + kc];
-        val += 2.0 * (yy[kc].x * tw.x - yy[kc].y * tw.y);
+    // along z (a), only block rows a' in [0,np); line (kb, kc)
+    for (int e = lane; e < N * HH; e += 64) {
+      double2 x[N];
+#pragma unroll
+      for (int ka = 0; ka < N; ka++) x[ka] = load_c(in + ka * N * HH + e);
+#pragma unroll
+      for (int a = 0; a < NP; a++) {
+        double2 acc = make_double2(0, 0);
+#pragma unroll
+        for (int ka = 0; ka < N; ka++) acc = cadd(acc, cmul(x[ka], w[((a + O) * ka) % N]));
+        y1[a * N * HH + e] = acc;
       }
-      dc[q * v.td + i] += val;
+    }
+    __syncthreads();
+    // along y (b), only b' in [0,np); line (a, kc)
+    for (int e = lane; e < NP * HH; e += 64) {
+      const int a = e / HH, kc = e % HH;
+      double2 x[N];
+#pragma unroll
+      for (int kb = 0; kb < N; kb++) x[kb] = y1[(a * N + kb) * HH + kc];
+#pragma unroll
+      for (int b = 0; b < NP; b++) {
+        double2 acc = make_double2(0, 0);
+#pragma unroll
+        for (int kb = 0; kb < N; kb++) acc = cadd(acc, cmul(x[kb], w[((b + O) * kb) % N]));
+        y2[(a * NP + b) * HH + kc] = acc;
+      }
+    }
+    __syncthreads();
+    // along x (c), c2r at the surface samples; line (a, b)
+    for (int ab = lane; ab < NP * NP; ab += 64) {
+      double2 yy[HH];
+#pragma unroll
+      for (int kc = 0; kc < HH; kc++) yy[kc] = y2[ab * HH + kc];
+#pragma unroll
+      for (int c = 0; c < NP; c++) {
+        const int q = qof[ab * NP + c];
+        if (q < 0) continue;
+        double val = yy[0].x + yy[N / 2].x * (((c + O) & 1) ? -1.0 : 1.0);
+#pragma unroll
+        for (int kc = 1; kc < N / 2; kc++) {
+          const double2 tw = w[((c + O) * kc) % N];
+          val += 2.0 * (yy[kc].x * tw.x - yy[kc].y * tw.y);
+        }
+        dc[q * v.td + i] += val;
+      }
     }
   }
+}
+
+template <typename C2>
+static hipError_t launch_fft_fwd_any(const VlistArgs& v, C2* dspec, hipStream_t s) {
+  const dim3 g(unsigned(v.nsrc)), b(64);
+  switch (v.np) {
+    case 3: k_fft_fwd<3, C2><<<g, b, 0, s>>>(v, dspec); break;
+    case 4: k_fft_fwd<4, C2><<<g, b, 0, s>>>(v, dspec); break;
+    case 5: k_fft_fwd<5, C2><<<g, b, 0, s>>>(v, dspec); break;
+    case 6: k_fft_fwd<6, C2><<<g, b, 0, s>>>(v, dspec); break;
+    case 7: k_fft_fwd<7, C2><<<g, b, 0, s>>>(v, dspec); break;
+    case 8: k_fft_fwd<8, C2><<<g, b, 0, s>>>(v, dspec); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <typename C2>
+static hipError_t launch_fft_inv_any(const VlistArgs& v, const C2* espec, hipStream_t s) {
+  const dim3 g(unsigned(v.ntrg)), b(64);
+  switch (v.np) {
+    case 3: k_fft_inv<3, C2><<<g, b, 0, s>>>(v, espec); break;
+    case 4: k_fft_inv<4, C2><<<g, b, 0, s>>>(v, espec); break;
+    case 5: k_fft_inv<5, C2><<<g, b, 0, s>>>(v, espec); break;
+    case 6: k_fft_inv<6, C2><<<g, b, 0, s>>>(v, espec); break;
+    case 7: k_fft_inv<7, C2><<<g, b, 0, s>>>(v, espec); break;
+    case 8: k_fft_inv<8, C2><<<g, b, 0, s>>>(v, espec); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_vlist(const VlistArgs& v, hipStream_t s) {
   if (v.ngrp <= 0) return hipSuccess;
   if (v.np > kMaxNp) return hipErrorInvalidValue;
-  k_fft_fwd<double2><<<dim3(unsigned(v.nsrc)), dim3(kFftThreads), 0, s>>>(v, v.dspec);
-  hipError_t e = hipGetLastError();
+  hipError_t e = launch_fft_fwd_any<double2>(v, v.dspec, s);
   if (e != hipSuccess) return e;
   const double nrm = 1.0 / double(v.n * v.n * v.n);
   dim3 grid(unsigned(v.ngrp), unsigned((v.H + 63) / 64));
@@ -433,22 +502,19 @@ hipError_t launch_vlist(const VlistArgs& v, hipStream_t s) {
   else return hipErrorInvalidValue;
   e = hipGetLastError();
   if (e != hipSuccess) return e;
-  k_fft_inv<double2><<<dim3(unsigned(v.ntrg)), dim3(kFftThreads), 0, s>>>(v, v.espec);
-  return hipGetLastError();
+  return launch_fft_inv_any<double2>(v, v.espec, s);
 }
 
 hipError_t launch_fft_fwd32(const VlistArgs& v, float2* dspec, hipStream_t s) {
   if (v.nsrc <= 0) return hipSuccess;
   if (v.np > kMaxNp) return hipErrorInvalidValue;
-  k_fft_fwd<float2><<<dim3(unsigned(v.nsrc)), dim3(kFftThreads), 0, s>>>(v, dspec);
-  return hipGetLastError();
+  return launch_fft_fwd_any<float2>(v, dspec, s);
 }
 
 hipError_t launch_fft_inv32(const VlistArgs& v, const float2* espec, hipStream_t s) {
   if (v.ntrg <= 0) return hipSuccess;
   if (v.np > kMaxNp) return hipErrorInvalidValue;
-  k_fft_inv<float2><<<dim3(unsigned(v.ntrg)), dim3(kFftThreads), 0, s>>>(v, espec);
-  return hipGetLastError();
+  return launch_fft_inv_any<float2>(v, espec, s);
 }
 
 // ---------------------------------------------------------------------------

@@ -405,6 +405,97 @@ hipError_t launch_hadamard_mixed(const VlistArgs& v, const float2* dspec, const 
   return hipGetLastError();
 }
 
-hipError_t launch_gemm_fma(const GemmArgs& g, hipStream_t s) { return launch_gemm_t<true>(g, s); }
+// ---------------------------------------------------------------------------
+// Mixed path: FP64 matrix cores (v_mfma_f64_16x16x4f64, FMA, no order
+// contract).  160 x 64 tile per workgroup (M = 152 or 296 needs 1 or 2 row
+// tiles, 5% or 8% padding); wave w owns columns [16w, 16w+16) and all ten
+// 16-row blocks; K staged through LDS in slabs of 16 with the next slab
+// prefetched into registers.  Operand layout of the 16x16x4 instruction:
+// lane l supplies A[l%16][l/16] and B[l/16][l%16] and its result register r
+// holds D[4r + l/16][l%16] (rows interleaved across the four lane groups).
+typedef double kd4 __attribute__((ext_vector_type(4)));
+constexpr int GM_M = 160, GM_N = 64, GM_K = 16, GM_MB = GM_M / 16;
+
+__global__ __launch_bounds__(256, 2) void k_gemm_mfma(GemmArgs g) {
+  __shared__ double As[GM_K][GM_M + 4];
+  __shared__ double Bs[GM_K][GM_N + 4];
+  __shared__ int64_t bcol[GM_N];
+  const int z = blockIdx.z;
+  const int cbeg = g.zoff ? g.zoff[z] : 0, cend = g.zoff ? g.zoff[z + 1] : g.ncol;
+  const int n0 = cbeg + int(blockIdx.y) * GM_N;
+  if (n0 >= cend) return;
+  const double* A = g.A + int64_t(z) * g.a_zstride;
+  const int m0 = blockIdx.x * GM_M;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
+  if (tid < GM_N) {
+    const int j = n0 + tid;
+    bcol[tid] = j < cend ? int64_t(g.bidx[j]) * g.ldb : int64_t(-1);
+  }
+  __syncthreads();
+  kd4 acc[GM_MB];
+#pragma unroll
+  for (int mi = 0; mi < GM_MB; mi++) acc[mi] = kd4{0.0, 0.0, 0.0, 0.0};
+  double ra[GM_MB], rb[4];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < GM_MB; u++) {
+      const int e = tid + 256 * u;
+      const int m = m0 + e % GM_M, k = k0 + e / GM_M;
+      ra[u] = (m < g.M && k < g.K) ? A[int64_t(k) * g.M + m] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int e = tid + 256 * u;
+      const int kq = k0 + e % GM_K;
+      const int64_t col = bcol[e / GM_K];
+      rb[u] = (col >= 0 && kq < g.K) ? g.B[col + kq] * g.kscale[kq % g.ksd] : 0.0;
+    }
+  };
+  fetch(0);
+  for (int k0 = 0; k0 < g.K; k0 += GM_K) {
+#pragma unroll
+    for (int u = 0; u < GM_MB; u++) {
+      const int e = tid + 256 * u;
+      As[e / GM_M][e % GM_M] = ra[u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int e = tid + 256 * u;
+      Bs[e % GM_K][e / GM_K] = rb[u];
+    }
+    __syncthreads();
+    if (k0 + GM_K < g.K) fetch(k0 + GM_K);
+#pragma unroll
+    for (int ks = 0; ks < GM_K / 4; ks++) {
+      const double b = Bs[4 * ks + lk][16 * wave + li];
+#pragma unroll
+      for (int mi = 0; mi < GM_MB; mi++)
+        acc[mi] = __builtin_amdgcn_mfma_f64_16x16x4f64(As[4 * ks + lk][16 * mi + li], b, acc[mi], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  const int n = n0 + 16 * wave + li;
+  if (n >= cend) return;
+  double* col = g.C + int64_t(g.cidx[n]) * g.ldc;
+#pragma unroll
+  for (int mi = 0; mi < GM_MB; mi++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int m = m0 + 16 * mi + 4 * r + lk;
+      if (m >= g.M) continue;
+      const double v = g.rscale[m % g.rsd] * acc[mi][r];
+      col[m] = g.accumulate ? col[m] + v : v;
+    }
+}
+
+static hipError_t launch_gemm_mfma(const GemmArgs& g, hipStream_t s) {
+  if (g.ncol <= 0) return hipSuccess;
+  const int nz = g.zoff ? g.nz : 1;
+  dim3 grid(unsigned((g.M + GM_M - 1) / GM_M), unsigned((g.max_zcol + GM_N - 1) / GM_N), unsigned(nz));
+  k_gemm_mfma<<<grid, dim3(256), 0, s>>>(g);
+  return hipGetLastError();
+}
+
+hipError_t launch_gemm_fma(const GemmArgs& g, hipStream_t s) { return launch_gemm_mfma(g, s); }
 
 }  // namespace kifmm
