@@ -138,6 +138,82 @@ int kifmm_evaluate(kifmm_handle h, const double* src_den, double* trg_val) {
   return KIFMM_OK;
 }
 
+int kifmm_evaluate_multi_device(kifmm_handle h, const double* d_src_den, double* d_trg_val, int nrhs,
+                                void* stream) {
+  if (!h) return set_err(nullptr, KIFMM_ERR_ARG, "null handle");
+  if (nrhs < 0) return set_err(h, KIFMM_ERR_ARG, "negative nrhs");
+  auto& f = h->fmm;
+  const int64_t nd = f.numSrc() * f.srcDOF(), nv = f.numTrg() * f.trgDOF();
+  for (int r = 0; r < nrhs; r++) {
+    int rc = f.evaluate(d_src_den + r * nd, d_trg_val + r * nv, static_cast<hipStream_t>(stream));
+    if (rc) return rc;
+  }
+  return KIFMM_OK;
+}
+
+int kifmm_evaluate_multi(kifmm_handle h, const double* src_den, double* trg_val, int nrhs) {
+  if (!h) return set_err(nullptr, KIFMM_ERR_ARG, "null handle");
+  auto& f = h->fmm;
+  if (!f.isSetup()) return set_err(h, KIFMM_ERR_STATE, "evaluate called before setup");
+  if (nrhs < 0) return set_err(h, KIFMM_ERR_ARG, "negative nrhs");
+  if (nrhs == 0) return KIFMM_OK;
+  if ((f.numSrc() > 0 && !src_den) || (f.numTrg() > 0 && !trg_val)) return set_err(h, KIFMM_ERR_ARG, "null array");
+  const size_t nd = size_t(f.numSrc()) * f.srcDOF() * nrhs, nv = size_t(f.numTrg()) * f.trgDOF() * nrhs;
+  kifmm::DeviceBuffer d, v;
+  hipError_t e;
+  if ((e = d.upload(src_den, sizeof(double) * nd)) != hipSuccess) return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
+  if ((e = v.alloc(sizeof(double) * nv)) != hipSuccess) return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
+  int rc = kifmm_evaluate_multi_device(h, d.as<double>(), v.as<double>(), nrhs, nullptr);
+  if (rc) return rc;
+  if ((e = hipMemcpy(trg_val, v.as<double>(), sizeof(double) * nv, hipMemcpyDeviceToHost)) != hipSuccess)
+    return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
+  return KIFMM_OK;
+}
+
+int kifmm_evaluate_adjoint_device(kifmm_handle h, const double* d_trg_in, double* d_src_out, void* stream) {
+  if (!h) return set_err(nullptr, KIFMM_ERR_ARG, "null handle");
+  return h->fmm.evaluateAdjoint(d_trg_in, d_src_out, static_cast<hipStream_t>(stream));
+}
+
+int kifmm_evaluate_adjoint(kifmm_handle h, const double* trg_in, double* src_out) {
+  if (!h) return set_err(nullptr, KIFMM_ERR_ARG, "null handle");
+  auto& f = h->fmm;
+  if (!f.isSetup()) return set_err(h, KIFMM_ERR_STATE, "evaluate_adjoint called before setup");
+  if ((f.numTrg() > 0 && !trg_in) || (f.numSrc() > 0 && !src_out)) return set_err(h, KIFMM_ERR_ARG, "null array");
+  const size_t nw = size_t(f.numTrg()) * f.trgDOF(), ns = size_t(f.numSrc()) * f.srcDOF();
+  kifmm::DeviceBuffer w, o;
+  hipError_t e;
+  if ((e = w.upload(trg_in, sizeof(double) * nw)) != hipSuccess) return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
+  if ((e = o.alloc(sizeof(double) * ns)) != hipSuccess) return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
+  int rc = f.evaluateAdjoint(w.as<double>(), o.as<double>(), nullptr);
+  if (rc) return rc;
+  if ((e = hipMemcpy(src_out, o.as<double>(), sizeof(double) * ns, hipMemcpyDeviceToHost)) != hipSuccess)
+    return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
+  return KIFMM_OK;
+}
+
+int kifmm_field_device(kifmm_handle h, const double* d_src_den, double* d_trg_grad, void* stream) {
+  if (!h) return set_err(nullptr, KIFMM_ERR_ARG, "null handle");
+  return h->fmm.field(d_src_den, d_trg_grad, static_cast<hipStream_t>(stream));
+}
+
+int kifmm_field(kifmm_handle h, const double* src_den, double* trg_grad) {
+  if (!h) return set_err(nullptr, KIFMM_ERR_ARG, "null handle");
+  auto& f = h->fmm;
+  if (!f.isSetup()) return set_err(h, KIFMM_ERR_STATE, "field called before setup");
+  if ((f.numSrc() > 0 && !src_den) || (f.numTrg() > 0 && !trg_grad)) return set_err(h, KIFMM_ERR_ARG, "null array");
+  const size_t nd = size_t(f.numSrc()) * f.srcDOF(), ng = size_t(f.numTrg()) * 3;
+  kifmm::DeviceBuffer d, g;
+  hipError_t e;
+  if ((e = d.upload(src_den, sizeof(double) * nd)) != hipSuccess) return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
+  if ((e = g.alloc(sizeof(double) * ng)) != hipSuccess) return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
+  int rc = f.field(d.as<double>(), g.as<double>(), nullptr);
+  if (rc) return rc;
+  if ((e = hipMemcpy(trg_grad, g.as<double>(), sizeof(double) * ng, hipMemcpyDeviceToHost)) != hipSuccess)
+    return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
+  return KIFMM_OK;
+}
+
 int kifmm_direct_device(kifmm_handle h, const double* d_src_den, int nchk, const double* d_chk_pos, double* d_chk_val,
                         void* stream) {
   if (!h) return set_err(nullptr, KIFMM_ERR_ARG, "null handle");

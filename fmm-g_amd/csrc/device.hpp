@@ -56,6 +56,9 @@ struct GemmArgs {
   // several matrices in one launch: matrix z is A + z*a_zstride and owns the
   // columns [zoff[z], zoff[z+1]); zoff == nullptr means one matrix, all columns
   const int* zoff; int nz; int64_t a_zstride; int max_zcol;
+  // split-K (mixed path only, zoff == nullptr): workgroup z sums k in
+  // [z*kchunk, (z+1)*kchunk) and adds its partial product to C atomically
+  int ksplit; int kchunk;
 };
 
 struct VlistArgs {
@@ -75,6 +78,40 @@ struct VlistArgs {
   double2* dspec;              // [nsrc][sd][H]
   double2* espec;              // [ntrg][td][H]
 };
+
+// Adjoint near field (source-centric): chunk = <= kNearChunk sources of a
+// source leaf (node, tbeg = first source in leaf order, tnum); items are
+// (node << 2) | kind over the points that feed them.
+enum AdjItemKind { ADJ_TRG = 0, ADJ_DC = 1, ADJ_UC = 2 };
+struct NearTArgs {
+  const NearChunk* chunks; int nchunks;
+  const int* items;
+  const double* spos; const double* snor;            // sources, leaf order
+  const double* tpos; const double* tw;              // targets, leaf order, adjoint input (TD each)
+  const int* trg_beg; const int* trg_num;
+  const double4* geom;
+  const double* sam_dc; int n_dc; const double* DC; int sz_dc;
+  const double* sam_uc; int n_uc; const double* UC; int sz_uc;
+  double* sout;                                      // SD per source, leaf order
+};
+
+// Adjoint of M2T / L2T: the equivalent surface of `node` receives from the
+// targets of the target leaves items[ibeg..iend).
+struct SurfTArgs {
+  const SurfTask* tasks; int ntasks;
+  const int* items;
+  const double* tpos; const double* tw; const int* trg_beg; const int* trg_num;
+  const double4* geom;
+  const double* sam; int n_sam;
+  double* out; int sz_out;
+};
+
+// Gradient (with respect to the target) of the near-field and local-expansion
+// terms, 3 values per target written to a.tval; Laplace single layer.
+hipError_t launch_near_grad(int kt, const NearArgs& a, const KParams<double>& pu, const KParams<double>& pm,
+                            hipStream_t s);
+hipError_t launch_near_adjoint(int kt, const NearTArgs& a, const KParams<double>& pu, hipStream_t s);
+hipError_t launch_surface_adjoint(int km, const SurfTArgs& a, const KParams<double>& pm, hipStream_t s);
 
 hipError_t launch_gather(const double* in, const int* perm, double* out, int64_t n, int dof, hipStream_t s);
 hipError_t launch_scatter(const double* in, const int* perm, double* out, int64_t n, int dof, hipStream_t s);

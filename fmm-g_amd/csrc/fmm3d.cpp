@@ -230,14 +230,36 @@ int FMM3d::build_tasks() {
   }
 
   // --- V list as sibling groups ---
-  vsrc_node_h_.clear(); vtrg_node_h_.clear(); vhalo_h_.clear(); vhord_h_.clear(); vtgt_h_.clear();
-  vpsrc_h_.clear(); vpslot_h_.clear(); vpcnt_h_.clear();
-  vmask_h_.clear();
+  std::vector<int> by_depth_order;
+  by_depth_order.reserve(N);
+  for (const auto& lv : by_depth) by_depth_order.insert(by_depth_order.end(), lv.begin(), lv.end());
+  int rc = build_vgroups(by_depth_order, t.off[LIST_V], t.lst[LIST_V], true, vfwd_);
+  if (rc) return rc;
+  work_.vlist_pairs = vfwd_.pairs;
+  work_.n_vsrc = double(vfwd_.src_node.size());
+  work_.n_vtrg = double(vfwd_.trg_node.size());
+  work_.n_vgrp = double(vfwd_.ngrp());
+  adj_ready_ = false;
+  return 0;
+}
+
+// Groups the V lists (CSR over nodes) of the nodes in `order` by parent; the
+// sources get slots in order of first use.  need_evtr: only evaluator nodes
+// are targets (the forward lists); the adjoint passes the inverted lists.
+int FMM3d::build_vgroups(const std::vector<int>& order, const std::vector<int64_t>& off,
+                         const std::vector<int>& lst, bool need_evtr, VGroups& vg) {
+  const Tree& t = tree_;
+  const int N = t.n_nodes;
+  vg.src_node.clear(); vg.trg_node.clear(); vg.halo.clear(); vg.hord.clear(); vg.tgt.clear();
+  vg.psrc.clear(); vg.pslot.clear(); vg.pcnt.clear(); vg.mask.clear();
+  vg.pairs = 0;
   std::vector<int> src_slot(N, -1);
   std::map<int, int> grp;
-  for (int B = 0; B < N; B++) {
-    if (!(t.tag[B] & TAG_EVTR)) continue;
-    const int64_t b0 = t.off[LIST_V][B], b1 = t.off[LIST_V][B + 1];
+  // level-major order (depth, then node index): consecutive groups, and the
+  // sources they first touch, are neighbours on one level
+  for (int B : order) {
+    if (need_evtr && !(t.tag[B] & TAG_EVTR)) continue;
+    const int64_t b0 = off[B], b1 = off[B + 1];
     if (b0 == b1) continue;
     const int P = t.parent[B], tau = B - t.child[P];
     auto it = grp.find(P);
@@ -245,76 +267,73 @@ int FMM3d::build_tasks() {
     if (it == grp.end()) {
       g = int(grp.size());
       grp[P] = g;
-      vhalo_h_.resize(size_t(g + 1) * 216, -1);
-      vtgt_h_.resize(size_t(g + 1) * 8, -1);
-      vmask_h_.resize(size_t(g + 1) * 32, 0);
+      vg.halo.resize(size_t(g + 1) * 216, -1);
+      vg.tgt.resize(size_t(g + 1) * 8, -1);
+      vg.mask.resize(size_t(g + 1) * 32, 0);
     } else g = it->second;
-    vtgt_h_[size_t(g) * 8 + tau] = int(vtrg_node_h_.size());
-    vtrg_node_h_.push_back(B);
+    vg.tgt[size_t(g) * 8 + tau] = int(vg.trg_node.size());
+    vg.trg_node.push_back(B);
     for (int64_t q = b0; q < b1; q++) {
-      const int A = t.lst[LIST_V][q];
-      if (src_slot[A] < 0) { src_slot[A] = int(vsrc_node_h_.size()); vsrc_node_h_.push_back(A); }
+      const int A = lst[q];
+      if (src_slot[A] < 0) { src_slot[A] = int(vg.src_node.size()); vg.src_node.push_back(A); }
       int r = 0;
       for (int d = 0; d < 3; d++) {
         int rel = t.path[3 * A + d] - 2 * t.path[3 * P + d] + 2;
         if (rel < 0 || rel > 5) return fail(-6, "V-list member outside the parent's colleague halo");
         r = r * 6 + rel;
       }
-      int& h = vhalo_h_[size_t(g) * 216 + r];
+      int& h = vg.halo[size_t(g) * 216 + r];
       if (h >= 0 && h != src_slot[A]) return fail(-6, "inconsistent V-list halo");
       h = src_slot[A];
-      vmask_h_[size_t(g) * 32 + tau * 4 + (r >> 6)] |= (uint64_t(1) << (r & 63));
-      work_.vlist_pairs += 1;
+      vg.mask[size_t(g) * 32 + tau * 4 + (r >> 6)] |= (uint64_t(1) << (r & 63));
+      vg.pairs += 1;
     }
   }
-  // per group: halo slots ordered by node index (the V lists' order)
   const int ngrp = int(grp.size());
   // The halo-sweep Hadamard relies on V(tau) = {occupied halo slots not adjacent
   // to tau} for every target child tau of a group; check it on the lists.
   for (int g = 0; g < ngrp; g++)
     for (int tau = 0; tau < 8; tau++) {
-      if (vtgt_h_[size_t(g) * 8 + tau] < 0) continue;
+      if (vg.tgt[size_t(g) * 8 + tau] < 0) continue;
       const int tc[3] = {tau >> 2, (tau >> 1) & 1, tau & 1};
       for (int r = 0; r < 216; r++) {
         const int rc[3] = {r / 36 - 2, (r / 6) % 6 - 2, r % 6 - 2};
         bool adj = true;
         for (int d = 0; d < 3; d++) adj = adj && std::abs(rc[d] - tc[d]) <= 1;
-        const bool in_v = (vmask_h_[size_t(g) * 32 + tau * 4 + (r >> 6)] >> (r & 63)) & 1;
-        if (in_v != (!adj && vhalo_h_[size_t(g) * 216 + r] >= 0))
+        const bool in_v = (vg.mask[size_t(g) * 32 + tau * 4 + (r >> 6)] >> (r & 63)) & 1;
+        if (in_v != (!adj && vg.halo[size_t(g) * 216 + r] >= 0))
           return fail(-6, "V list differs from the non-adjacent halo of its sibling group");
       }
     }
-  vhord_h_.assign(size_t(ngrp) * 216, -1);
+  // per group: halo slots ordered by node index (the V lists' order)
+  vg.hord.assign(size_t(ngrp) * 216, -1);
   for (auto& pg : grp) {
     const int g = pg.second;
     std::vector<std::pair<int, int>> nodes;
     for (int r = 0; r < 216; r++) {
-      int sl = vhalo_h_[size_t(g) * 216 + r];
-      if (sl >= 0) nodes.push_back({vsrc_node_h_[sl], r});
+      int sl = vg.halo[size_t(g) * 216 + r];
+      if (sl >= 0) nodes.push_back({vg.src_node[sl], r});
     }
     std::sort(nodes.begin(), nodes.end());
-    for (size_t k = 0; k < nodes.size(); k++) vhord_h_[size_t(g) * 216 + k] = nodes[k].second;
+    for (size_t k = 0; k < nodes.size(); k++) vg.hord[size_t(g) * 216 + k] = nodes[k].second;
   }
-  vpsrc_h_.assign(size_t(ngrp) * kPairStride, -1);
-  vpslot_h_.assign(size_t(ngrp) * kPairStride, 0);
-  vpcnt_h_.assign(ngrp, 0);
+  vg.psrc.assign(size_t(ngrp) * kPairStride, -1);
+  vg.pslot.assign(size_t(ngrp) * kPairStride, 0);
+  vg.pcnt.assign(ngrp, 0);
   for (int g = 0; g < ngrp; g++) {
     int k = 0;
-    for (; k < 216 && vhord_h_[size_t(g) * 216 + k] >= 0; k++) {
-      const int r = vhord_h_[size_t(g) * 216 + k];
+    for (; k < 216 && vg.hord[size_t(g) * 216 + k] >= 0; k++) {
+      const int r = vg.hord[size_t(g) * 216 + k];
       const int rx = r / 36, ry = (r / 6) % 6, rz = r % 6;  // halo coordinate + 2
       const int slot = (rx + 1) * 49 + (ry + 1) * 7 + (rz + 1);  // offset (r - 2) + 3
       int tm = 0;  // targets of the group whose V list holds this source
-      for (int t = 0; t < 8; t++)
-        if ((vmask_h_[size_t(g) * 32 + t * 4 + (r >> 6)] >> (r & 63)) & 1) tm |= 1 << t;
-      vpsrc_h_[size_t(g) * kPairStride + k] = vhalo_h_[size_t(g) * 216 + r];
-      vpslot_h_[size_t(g) * kPairStride + k] = r | slot << 8 | tm << 20;
+      for (int tt = 0; tt < 8; tt++)
+        if ((vg.mask[size_t(g) * 32 + tt * 4 + (r >> 6)] >> (r & 63)) & 1) tm |= 1 << tt;
+      vg.psrc[size_t(g) * kPairStride + k] = vg.halo[size_t(g) * 216 + r];
+      vg.pslot[size_t(g) * kPairStride + k] = r | slot << 8 | tm << 20;
     }
-    vpcnt_h_[g] = (k + 7) / 8 * 8;
+    vg.pcnt[g] = (k + 7) / 8 * 8;
   }
-  work_.n_vsrc = double(vsrc_node_h_.size());
-  work_.n_vtrg = double(vtrg_node_h_.size());
-  work_.n_vgrp = double(grp.size());
   return 0;
 }
 
@@ -392,25 +411,20 @@ int FMM3d::upload_state() {
   KIFMM_UP(l2l_d_, op.l2l, "upload L2L");
   KIFMM_UP(m2l_d_, op.m2l, "upload M2L");
   // V list
-  KIFMM_UP(vsrc_d_, vsrc_node_h_, "upload V sources");
-  KIFMM_UP(vtrg_d_, vtrg_node_h_, "upload V targets");
-  KIFMM_UP(vhalo_d_, vhalo_h_, "upload V halo");
-  KIFMM_UP(vhord_d_, vhord_h_, "upload V order");
-  KIFMM_UP(vpsrc_d_, vpsrc_h_, "upload V pairs");
-  KIFMM_UP(vpslot_d_, vpslot_h_, "upload V pairs");
-  KIFMM_UP(vpcnt_d_, vpcnt_h_, "upload V pairs");
-  KIFMM_UP(vtgt_d_, vtgt_h_, "upload V groups");
-  KIFMM_UP(vmask_d_, vmask_h_, "upload V masks");
-  if (precision_ == PREC_FP64) {
-    if ((rc = check(dspec_d_.alloc(sizeof(double2) * vsrc_node_h_.size() * op.sd * op.H), "alloc V spectra"))) return rc;
-    if ((rc = check(espec_d_.alloc(sizeof(double2) * vtrg_node_h_.size() * op.td * op.H), "alloc V spectra"))) return rc;
-    m2l32_d_.release(); dspec32_d_.release(); espec32_d_.release();
-  } else {
-    std::vector<float> m2l32(op.m2l.begin(), op.m2l.end());
-    KIFMM_UP(m2l32_d_, m2l32, "upload M2L (FP32)");
-    if ((rc = check(dspec32_d_.alloc(sizeof(float2) * vsrc_node_h_.size() * op.sd * op.H), "alloc V spectra"))) return rc;
-    if ((rc = check(espec32_d_.alloc(sizeof(float2) * vtrg_node_h_.size() * op.td * op.H), "alloc V spectra"))) return rc;
-    dspec_d_.release(); espec_d_.release();
+  if ((rc = upload_vgroups(vfwd_))) return rc;
+  {
+    const size_t ns = vfwd_.src_node.size() * op.sd, nt = vfwd_.trg_node.size() * op.td;
+    if (precision_ == PREC_FP64) {
+      if ((rc = check(dspec_d_.alloc(sizeof(double2) * ns * op.H), "alloc V spectra"))) return rc;
+      if ((rc = check(espec_d_.alloc(sizeof(double2) * nt * op.H), "alloc V spectra"))) return rc;
+      m2l32_d_.release(); dspec32_d_.release(); espec32_d_.release();
+    } else {
+      std::vector<float> m2l32(op.m2l.begin(), op.m2l.end());
+      KIFMM_UP(m2l32_d_, m2l32, "upload M2L (FP32)");
+      if ((rc = check(dspec32_d_.alloc(sizeof(float2) * ns * op.H), "alloc V spectra"))) return rc;
+      if ((rc = check(espec32_d_.alloc(sizeof(float2) * nt * op.H), "alloc V spectra"))) return rc;
+      dspec_d_.release(); espec_d_.release();
+    }
   }
 #undef KIFMM_UP
   if (events_.empty()) {
@@ -421,11 +435,71 @@ int FMM3d::upload_state() {
   return 0;
 }
 
+int FMM3d::upload_vgroups(VGroups& vg) {
+  int rc;
+#define KIFMM_UPV(buf, vec, what) \
+  if ((rc = check((buf).upload((vec).data(), sizeof((vec)[0]) * (vec).size()), what))) return rc;
+  KIFMM_UPV(vg.src_node_d, vg.src_node, "upload V sources");
+  KIFMM_UPV(vg.trg_node_d, vg.trg_node, "upload V targets");
+  KIFMM_UPV(vg.halo_d, vg.halo, "upload V halo");
+  KIFMM_UPV(vg.hord_d, vg.hord, "upload V order");
+  KIFMM_UPV(vg.psrc_d, vg.psrc, "upload V pairs");
+  KIFMM_UPV(vg.pslot_d, vg.pslot, "upload V pairs");
+  KIFMM_UPV(vg.pcnt_d, vg.pcnt, "upload V pairs");
+  KIFMM_UPV(vg.tgt_d, vg.tgt, "upload V groups");
+  KIFMM_UPV(vg.mask_d, vg.mask, "upload V masks");
+#undef KIFMM_UPV
+  return 0;
+}
+
+// Group tables and operators of a V pass; the caller sets the input
+// (UE, sz_ue: equivalent densities of the sources) and the output (DC, sz_dc:
+// check values of the targets, added to).
+void FMM3d::vlist_args(VlistArgs& v, const VGroups& vg) const {
+  const Operators& op = *ops_;
+  v.np = op.np; v.n = op.n; v.H = op.H; v.sd = op.sd; v.td = op.td;
+  v.nsrc = int(vg.src_node.size()); v.ntrg = int(vg.trg_node.size()); v.ngrp = vg.ngrp();
+  v.src_node = vg.src_node_d.as<int>(); v.trg_node = vg.trg_node_d.as<int>();
+  v.depth = depth_d_.as<int>(); v.root_level = tree_.root_level;
+  v.deg = deg_d_.as<double>(); v.shell = shell_d_.as<int>(); v.twiddle = twiddle_d_.as<double2>();
+  v.halo = vg.halo_d.as<int>(); v.hord = vg.hord_d.as<int>();
+  v.psrc = vg.psrc_d.as<int>(); v.pslot = vg.pslot_d.as<int>(); v.pcnt = vg.pcnt_d.as<int>();
+  v.mask = vg.mask_d.as<uint64_t>(); v.tgt = vg.tgt_d.as<int>();
+  v.m2l = m2l_d_.as<double2>(); v.dspec = dspec_d_.as<double2>(); v.espec = espec_d_.as<double2>();
+}
+
+int FMM3d::run_vlist(VlistArgs& v, hipStream_t s) {
+  int rc;
+  if (v.ngrp <= 0) return 0;
+  if (precision_ != PREC_MIXED) return check(launch_vlist(v, s), "V list");
+  if ((rc = check(launch_fft_fwd32(v, dspec32_d_.as<float2>(), s), "V list forward"))) return rc;
+  if ((rc = check(launch_hadamard_mixed(v, dspec32_d_.as<float2>(), m2l32_d_.as<float2>(), espec32_d_.as<float2>(), s),
+                  "V list Hadamard")))
+    return rc;
+  return check(launch_fft_inv32(v, espec32_d_.as<float2>(), s), "V list inverse");
+}
+
 // ---------------------------------------------------------------------------
 // evaluate: fmm3d_eval_mpi.cpp:396-1515
 // ---------------------------------------------------------------------------
 int FMM3d::evaluate(const double* d_src_den, double* d_trg_val, hipStream_t s) {
   if (!setup_done_) return fail(-3, "evaluate called before setup");
+  return evaluate_impl(d_src_den, d_trg_val, s, false);
+}
+
+int FMM3d::field(const double* d_src_den, double* d_trg_grad, hipStream_t s) {
+  if (!setup_done_) return fail(-3, "field called before setup");
+  if (kt_ != 111) return fail(-2, "the field pass supports the Laplace single layer (kernel 111) only");
+  if (tgrad_d_.bytes() < sizeof(double) * 3 * size_t(std::max<int64_t>(1, nt_))) {
+    int rc = check(tgrad_d_.alloc(sizeof(double) * 3 * size_t(std::max<int64_t>(1, nt_))), "alloc field");
+    if (rc) return rc;
+  }
+  return evaluate_impl(d_src_den, d_trg_grad, s, true);
+}
+
+// grad: the same far-field pass, then the gradient of the near-field and local
+// expansion kernels at the targets (3 values per target) instead of the values.
+int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t s, bool grad) {
   if ((ns_ > 0 && !d_src_den) || (nt_ > 0 && !d_trg_val)) return fail(-2, "null density or value pointer");
   const Tree& t = tree_;
   const Operators& op = *ops_;
@@ -479,6 +553,9 @@ int FMM3d::evaluate(const double* d_src_den, double* d_trg_val, hipStream_t s) {
       if (st.rs_off >= 0) { g.rscale = scales_d_.as<double>() + st.rs_off; g.rsd = op.sd; }
       else { g.rscale = ones_d_.as<double>(); g.rsd = 1; }
       g.ncol = st.ncol; g.accumulate = st.accumulate;
+      // mixed path: the M2M sum over the 8 children runs as 8 K chunks whose
+      // partial products are added atomically into the zeroed parent UC
+      if (mixed && st.mat == 0) { g.ksplit = 8; g.kchunk = op.dat(UE); }
       int r = check(mixed ? launch_gemm_fma(g, s) : launch_gemm(g, s), "level GEMM");
       if (r) return r;
     }
@@ -496,24 +573,10 @@ int FMM3d::evaluate(const double* d_src_den, double* d_trg_val, hipStream_t s) {
   // V list (:1007-1059)
   mark(PH_VLIST);
   VlistArgs v{};
-  v.np = op.np; v.n = op.n; v.H = op.H; v.sd = op.sd; v.td = op.td;
-  v.nsrc = int(vsrc_node_h_.size()); v.ntrg = int(vtrg_node_h_.size()); v.ngrp = int(vtgt_h_.size() / 8);
-  v.src_node = vsrc_d_.as<int>(); v.trg_node = vtrg_d_.as<int>();
-  v.depth = depth_d_.as<int>(); v.root_level = t.root_level;
-  v.deg = deg_d_.as<double>(); v.shell = shell_d_.as<int>(); v.twiddle = twiddle_d_.as<double2>();
+  vlist_args(v, vfwd_);
   v.UE = ue_d_.as<double>(); v.sz_ue = op.dat(UE);
   v.DC = dc_d_.as<double>(); v.sz_dc = op.dat(DC);
-  v.halo = vhalo_d_.as<int>(); v.hord = vhord_d_.as<int>();
-  v.psrc = vpsrc_d_.as<int>(); v.pslot = vpslot_d_.as<int>(); v.pcnt = vpcnt_d_.as<int>(); v.mask = vmask_d_.as<uint64_t>(); v.tgt = vtgt_d_.as<int>();
-  v.m2l = m2l_d_.as<double2>(); v.dspec = dspec_d_.as<double2>(); v.espec = espec_d_.as<double2>();
-  if (!mixed) {
-    if ((rc = check(launch_vlist(v, s), "V list"))) return rc;
-  } else if (v.ngrp > 0) {
-    if ((rc = check(launch_fft_fwd32(v, dspec32_d_.as<float2>(), s), "V list forward"))) return rc;
-    if ((rc = check(launch_hadamard_mixed(v, dspec32_d_.as<float2>(), m2l32_d_.as<float2>(), espec32_d_.as<float2>(), s),
-                    "V list Hadamard"))) return rc;
-    if ((rc = check(launch_fft_inv32(v, espec32_d_.as<float2>(), s), "V list inverse"))) return rc;
-  }
+  if ((rc = run_vlist(v, s))) return rc;
 
   // X list S2L (:1233-1265)
   mark(PH_XLIST);
@@ -539,12 +602,19 @@ int FMM3d::evaluate(const double* d_src_den, double* d_trg_val, hipStream_t s) {
   na.UE = ue_d_.as<double>(); na.sz_ue = op.dat(UE);
   na.DE = de_d_.as<double>(); na.sz_de = op.dat(DE);
   na.tpos = tpos_d_.as<double>(); na.tval = tval_d_.as<double>();
-  if ((rc = check(mixed ? launch_near_mixed(kt_, na, pu, pm, s) : launch_near(kt_, na, pu, pm, s), "near field")))
+  if (grad) {
+    na.tval = tgrad_d_.as<double>();
+    if ((rc = check(launch_near_grad(kt_, na, pu, pm, s), "near field gradient"))) return rc;
+  } else if ((rc = check(mixed ? launch_near_mixed(kt_, na, pu, pm, s) : launch_near(kt_, na, pu, pm, s),
+                         "near field"))) {
     return rc;
+  }
 
   // write back (:1490-1510)
   mark(PH_FIN);
-  if ((rc = check(launch_scatter(tval_d_.as<double>(), tperm_d_.as<int>(), d_trg_val, nt_, TD, s), "scatter"))) return rc;
+  if ((rc = check(launch_scatter(grad ? tgrad_d_.as<double>() : tval_d_.as<double>(), tperm_d_.as<int>(), d_trg_val,
+                                 nt_, grad ? 3 : TD, s), "scatter")))
+    return rc;
   mark(PH_COUNT);
   if (timing_) {
     if ((rc = check(hipEventSynchronize(events_[PH_COUNT]), "sync"))) return rc;

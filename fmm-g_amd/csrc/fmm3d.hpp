@@ -40,6 +40,18 @@ class DeviceBuffer {
   size_t bytes_ = 0;
 };
 
+// V-list interactions as sibling groups: the 8 children of one parent and
+// their parent's 6^3 halo of colleague children.  Per group: halo slot ->
+// source slot (-1 empty), target child -> target slot, V membership bits, and
+// the halo slots in node order as pair lists for the ordered FP64 kernel.
+struct VGroups {
+  std::vector<int> src_node, trg_node, halo, hord, tgt, psrc, pslot, pcnt;
+  std::vector<uint64_t> mask;
+  double pairs = 0;
+  DeviceBuffer src_node_d, trg_node_d, halo_d, hord_d, tgt_d, psrc_d, pslot_d, pcnt_d, mask_d;
+  int ngrp() const { return int(tgt.size() / 8); }
+};
+
 class FMM3d {
  public:
   FMM3d() = default;
@@ -67,6 +79,13 @@ class FMM3d {
   int setup();
   // Device pointers, caller's point order (srcDOF / trgDOF interleaved).
   int evaluate(const double* d_src_den, double* d_trg_val, hipStream_t stream);
+  // Transposed pass: d_src_out (srcDOF per source) = A^T d_trg_in (trgDOF per
+  // target), A the evaluate operator.  Needs a symmetric equivalent kernel
+  // (111 or 511); the adjoint tables are built on the first call.
+  int evaluateAdjoint(const double* d_trg_in, double* d_src_out, hipStream_t stream);
+  // Gradient of the evaluated potential at the targets (3 values per target,
+  // caller's order); Laplace single layer only.
+  int field(const double* d_src_den, double* d_trg_grad, hipStream_t stream);
   // FP64 direct sum at the given targets (device pointers, caller's order).
   int direct(const double* d_src_den, int nchk, const double* d_chk_pos, double* d_chk_val,
              hipStream_t stream);
@@ -93,8 +112,15 @@ class FMM3d {
  private:
   int fail(int code, const std::string& msg) { err_ = msg; return code; }
   int check(hipError_t e, const char* what);
+  int evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t s, bool grad);
   int build_tasks();
   int upload_state();
+  int build_vgroups(const std::vector<int>& order, const std::vector<int64_t>& off, const std::vector<int>& lst,
+                    bool need_evtr, VGroups& vg);
+  int upload_vgroups(VGroups& vg);
+  void vlist_args(VlistArgs& v, const VGroups& vg) const;
+  int run_vlist(VlistArgs& v, hipStream_t s);
+  int build_adjoint();
 
   int kt_ = 111, np_ = 6, pts_max_ = 64, max_level_ = 30, root_level_ = 1, precision_ = PREC_FP64;
   std::vector<double> coefs_{1.0, 0.25};
@@ -111,7 +137,9 @@ class FMM3d {
   // host task tables
   // One level-batched GEMM launch of the up or down pass.  mat: 0 = M2M
   // (the 8 child matrices side by side), 1 = UC->UE pinv, 2 = DC->DE pinv,
-  // 3+c = L2L for child c.  src/dst: 0 = UE, 1 = UC, 2 = DE, 3 = DC.
+  // 3 = L2L (the 8 child matrices, one per z slice); transposed tables of the
+  // adjoint: 4 = M2M^T (stacked, 8*UE x UC), 5 = pinv_up^T, 6 = pinv_dn^T,
+  // 7 = L2L^T (side by side, DE x 8*DC).  src/dst: 0 = UE, 1 = UC, 2 = DE, 3 = DC.
   struct GemmStep {
     int mat, src, dst, ncol, accumulate;
     int64_t idx_off;  // bidx at idx_off, cidx at idx_off + ncol
@@ -126,8 +154,14 @@ class FMM3d {
   std::vector<int> near_items_h_;
   std::vector<SurfTask> s2m_tasks_h_, s2l_tasks_h_;
   std::vector<int> surf_items_h_;
-  std::vector<int> vsrc_node_h_, vtrg_node_h_, vhalo_h_, vhord_h_, vtgt_h_, vpsrc_h_, vpslot_h_, vpcnt_h_;
-  std::vector<uint64_t> vmask_h_;
+  VGroups vfwd_, vadj_;
+  // adjoint tables (built on the first evaluateAdjoint)
+  bool adj_ready_ = false;
+  std::vector<NearChunk> nt_chunks_h_;
+  std::vector<int> nt_items_h_;
+  std::vector<SurfTask> ue_t_tasks_h_, de_t_tasks_h_;
+  std::vector<int> surf_t_items_h_;
+  std::vector<GemmStep> up_t_steps_, dn_t_steps_;
 
   // device state
   DeviceBuffer spos_d_, snor_d_, tpos_d_, sperm_d_, tperm_d_, sden_d_, tval_d_;
@@ -137,7 +171,9 @@ class FMM3d {
   DeviceBuffer lvl_idx_d_;
   DeviceBuffer sam_d_[4], shell_d_, twiddle_d_, deg_d_, ones_d_, scales_d_;
   DeviceBuffer pup_d_, pdn_d_, m2m_d_, l2l_d_, m2l_d_;
-  DeviceBuffer vsrc_d_, vtrg_d_, vhalo_d_, vhord_d_, vtgt_d_, vpsrc_d_, vpslot_d_, vpcnt_d_, vmask_d_, dspec_d_, espec_d_;
+  DeviceBuffer dspec_d_, espec_d_, tgrad_d_;
+  DeviceBuffer nt_chunks_d_, nt_items_d_, ue_t_tasks_d_, de_t_tasks_d_, surf_t_items_d_, tbeg_d_, tnum_d_;
+  DeviceBuffer m2mT_d_, pupT_d_, pdnT_d_, l2lT_d_;
   DeviceBuffer direct_pos_d_, direct_nor_d_;
   DeviceBuffer m2l32_d_, dspec32_d_, espec32_d_;  // mixed path: FP32 spectra
   std::vector<hipEvent_t> events_;

@@ -166,6 +166,101 @@ __global__ __launch_bounds__(kNearChunk) void k_near_mx(NearArgs a, KParams<floa
   }
 }
 
+// Laplace single layer (user and equivalent-density kernel 111): sources staged
+// as float4 (x, y, z relative to the leaf centre, density * 1/(4 pi)), one
+// broadcast LDS read per pair; each block is padded to kNearChunk entries
+// with zero weight so lanes sweep contiguous ranges unrolled by 4.  The source
+// split S is a power of two (lanes = S copies of the chunk's targets), and
+// the FP32 partial sums per item are added in FP64 as in k_near_mx.
+__global__ __launch_bounds__(kNearChunk) void k_near_lap_mx(NearArgs a, float pa_u, float pa_m) {
+  __shared__ float4 s4[kNearChunk];
+  __shared__ double red[kNearChunk * 2];
+  const NearChunk ch = a.chunks[blockIdx.x];
+  const double4 cb = a.geom[ch.node];
+  const int lane = threadIdx.x;
+  const int tn = max(ch.tnum, 1);
+  int S = kNearChunk / tn;
+  S = S > 16 ? 16 : S;
+  S = 1 << (31 - __clz(S));
+  const int L = kNearChunk / S;  // multiple of 8
+  const int ti = lane % tn, sub = lane / tn;
+  const bool active = sub < S;
+  float tx = 0, ty = 0, tz = 0;
+  if (active) {
+    const double* p = a.tpos + 3 * int64_t(ch.tbeg + ti);
+    tx = float(p[0] - cb.x); ty = float(p[1] - cb.y); tz = float(p[2] - cb.z);
+  }
+  const float eps2 = float(kMinDif * kMinDif);
+  double near = 0.0, far = 0.0;
+  for (int it = ch.ibeg; it < ch.iend; it++) {
+    const int code = a.items[it];
+    const int node = code >> 2, kind = code & 3;
+    const bool exact = kind == ITEM_EXACT;
+    int ns, b = 0;
+    double4 g = make_double4(0, 0, 0, 0);
+    const double* sam = nullptr;
+    const double* den = nullptr;
+    if (exact) {
+      b = a.src_beg[node]; ns = a.src_num[node];
+    } else {
+      g = a.geom[node];
+      const bool ue = kind == ITEM_UE;
+      sam = ue ? a.sam_ue : a.sam_de;
+      ns = ue ? a.n_ue : a.n_de;
+      den = ue ? a.UE + int64_t(node) * a.sz_ue : a.DE + int64_t(node) * a.sz_de;
+    }
+    float acc0 = 0.f, acc1 = 0.f;
+    for (int base = 0; base < ns; base += kNearChunk) {
+      const int m = min(kNearChunk, ns - base);
+      __syncthreads();
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (lane < m) {
+        if (exact) {
+          const int64_t q = int64_t(b + base + lane);
+          v = make_float4(float(a.spos[3 * q] - cb.x), float(a.spos[3 * q + 1] - cb.y),
+                          float(a.spos[3 * q + 2] - cb.z), float(a.sden[q]) * pa_u);
+        } else {
+          const int q = base + lane;
+          v = make_float4(float((g.x - cb.x) + g.w * sam[3 * q]), float((g.y - cb.y) + g.w * sam[3 * q + 1]),
+                          float((g.z - cb.z) + g.w * sam[3 * q + 2]), float(den[q]) * pa_m);
+        }
+      }
+      s4[lane] = v;
+      __syncthreads();
+      if (active) {
+        const int j0 = sub * L, j1 = min(j0 + L, (m + 3) & ~3);
+        for (int j = j0; j < j1; j += 4) {
+#pragma unroll
+          for (int u = 0; u < 4; u += 2) {
+            const float4 s0 = s4[j + u], s1 = s4[j + u + 1];
+            const float dx0 = tx - s0.x, dy0 = ty - s0.y, dz0 = tz - s0.z;
+            const float dx1 = tx - s1.x, dy1 = ty - s1.y, dz1 = tz - s1.z;
+            const float r0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
+            const float r1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
+            acc0 = fmaf(r0 < eps2 ? 0.f : rsqrtf(r0), s0.w, acc0);
+            acc1 = fmaf(r1 < eps2 ? 0.f : rsqrtf(r1), s1.w, acc1);
+          }
+        }
+      }
+    }
+    if (kind == ITEM_DE) far += double(acc0 + acc1);
+    else near += double(acc0 + acc1);
+  }
+  __syncthreads();
+  if (active && sub > 0) {
+    red[(lane - tn) * 2] = near;
+    red[(lane - tn) * 2 + 1] = far;
+  }
+  __syncthreads();
+  if (lane < ch.tnum) {
+    for (int u = 1; u < S; u++) {
+      near += red[((u - 1) * tn + lane) * 2];
+      far += red[((u - 1) * tn + lane) * 2 + 1];
+    }
+    a.tval[int64_t(ch.tbeg + lane)] = near + far;
+  }
+}
+
 // ---------------------------------------------------------------------------
 constexpr int kSurfMx = 256;
 
@@ -366,6 +461,10 @@ hipError_t launch_near_mixed(int kt, const NearArgs& a, const KParams<double>& p
                              hipStream_t s) {
   if (a.nchunks <= 0) return hipSuccess;
   const KParams<float> fu = to_f32(pu), fm = to_f32(pm);
+  if (kt == 111) {
+    k_near_lap_mx<<<dim3(unsigned(a.nchunks)), dim3(kNearChunk), 0, s>>>(a, fu.a, fm.a);
+    return hipGetLastError();
+  }
   switch (kt) {
 #define KIFMM_CASE(K) case K: k_near_mx<K><<<dim3(unsigned(a.nchunks)), dim3(kNearChunk), 0, s>>>(a, fu, fm); break;
     KIFMM_USER_KERNELS_MX(KIFMM_CASE)
@@ -409,22 +508,32 @@ hipError_t launch_hadamard_mixed(const VlistArgs& v, const float2* dspec, const 
 // Mixed path: FP64 matrix cores (v_mfma_f64_16x16x4f64, FMA, no order
 // contract).  160 x 64 tile per workgroup (M = 152 or 296 needs 1 or 2 row
 // tiles, 5% or 8% padding); wave w owns columns [16w, 16w+16) and all ten
-// 16-row blocks; K staged through LDS in slabs of 16 with the next slab
-// prefetched into registers.  Operand layout of the 16x16x4 instruction:
-// lane l supplies A[l%16][l/16] and B[l/16][l%16] and its result register r
-// holds D[4r + l/16][l%16] (rows interleaved across the four lane groups).
+// 16-row blocks; K staged through LDS in slabs of 32 with the next slab
+// prefetched into registers while the current one feeds 80 MFMAs per wave.
+// With ksplit > 1, blockIdx.z picks a K chunk and the partial products are
+// added to C with FP64 atomics (the M2M of a level splits over the 8 children,
+// so a level with few parents still spreads over 8x the workgroups).  Operand
+// layout of the 16x16x4 instruction: lane l supplies A[l%16][l/16] and
+// B[l/16][l%16], and its result register r holds D[4r + l/16][l%16] (rows
+// interleaved across the four lane groups).
 typedef double kd4 __attribute__((ext_vector_type(4)));
-constexpr int GM_M = 160, GM_N = 64, GM_K = 16, GM_MB = GM_M / 16;
+constexpr int GM_M = 160, GM_N = 64, GM_K = 32, GM_MB = GM_M / 16;
+constexpr int GM_RA = GM_M * GM_K / 256, GM_RB = GM_K * GM_N / 256;
 
 __global__ __launch_bounds__(256, 2) void k_gemm_mfma(GemmArgs g) {
-  __shared__ double As[GM_K][GM_M + 4];
-  __shared__ double Bs[GM_K][GM_N + 4];
+  // row pitches = 16 mod 32 doubles: the four row groups of an operand read
+  // fall on disjoint bank halves
+  __shared__ double As[GM_K][GM_M + 16];
+  __shared__ double Bs[GM_K][GM_N + 16];
   __shared__ int64_t bcol[GM_N];
   const int z = blockIdx.z;
-  const int cbeg = g.zoff ? g.zoff[z] : 0, cend = g.zoff ? g.zoff[z + 1] : g.ncol;
+  const bool split = g.ksplit > 1;
+  const int cbeg = (!split && g.zoff) ? g.zoff[z] : 0, cend = (!split && g.zoff) ? g.zoff[z + 1] : g.ncol;
   const int n0 = cbeg + int(blockIdx.y) * GM_N;
   if (n0 >= cend) return;
-  const double* A = g.A + int64_t(z) * g.a_zstride;
+  const double* A = g.A + (split ? int64_t(0) : int64_t(z) * g.a_zstride);
+  const int kbeg = split ? z * g.kchunk : 0;
+  const int kend = split ? min(g.K, kbeg + g.kchunk) : g.K;
   const int m0 = blockIdx.x * GM_M;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
   if (tid < GM_N) {
@@ -435,36 +544,35 @@ __global__ __launch_bounds__(256, 2) void k_gemm_mfma(GemmArgs g) {
   kd4 acc[GM_MB];
 #pragma unroll
   for (int mi = 0; mi < GM_MB; mi++) acc[mi] = kd4{0.0, 0.0, 0.0, 0.0};
-  double ra[GM_MB], rb[4];
+  double ra[GM_RA], rb[GM_RB];
+  const double ks0 = g.kscale[0];
   auto fetch = [&](int k0) {
 #pragma unroll
-    for (int u = 0; u < GM_MB; u++) {
+    for (int u = 0; u < GM_RA; u++) {
       const int e = tid + 256 * u;
       const int m = m0 + e % GM_M, k = k0 + e / GM_M;
-      ra[u] = (m < g.M && k < g.K) ? A[int64_t(k) * g.M + m] : 0.0;
+      ra[u] = (m < g.M && k < kend) ? A[int64_t(k) * g.M + m] : 0.0;
     }
+    // B: lane owns column tid/4 and the 8 consecutive k from 8*(tid%4)
+    const int64_t col = bcol[tid >> 2];
+    const int kq0 = k0 + 8 * (tid & 3);
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int e = tid + 256 * u;
-      const int kq = k0 + e % GM_K;
-      const int64_t col = bcol[e / GM_K];
-      rb[u] = (col >= 0 && kq < g.K) ? g.B[col + kq] * g.kscale[kq % g.ksd] : 0.0;
+    for (int u = 0; u < GM_RB; u++) {
+      const int kq = kq0 + u;
+      rb[u] = (col >= 0 && kq < kend) ? g.B[col + kq] * (g.ksd == 1 ? ks0 : g.kscale[kq % g.ksd]) : 0.0;
     }
   };
-  fetch(0);
-  for (int k0 = 0; k0 < g.K; k0 += GM_K) {
+  fetch(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += GM_K) {
 #pragma unroll
-    for (int u = 0; u < GM_MB; u++) {
+    for (int u = 0; u < GM_RA; u++) {
       const int e = tid + 256 * u;
       As[e / GM_M][e % GM_M] = ra[u];
     }
 #pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int e = tid + 256 * u;
-      Bs[e % GM_K][e / GM_K] = rb[u];
-    }
+    for (int u = 0; u < GM_RB; u++) Bs[8 * (tid & 3) + u][tid >> 2] = rb[u];
     __syncthreads();
-    if (k0 + GM_K < g.K) fetch(k0 + GM_K);
+    if (k0 + GM_K < kend) fetch(k0 + GM_K);
 #pragma unroll
     for (int ks = 0; ks < GM_K / 4; ks++) {
       const double b = Bs[4 * ks + lk][16 * wave + li];
@@ -484,13 +592,16 @@ __global__ __launch_bounds__(256, 2) void k_gemm_mfma(GemmArgs g) {
       const int m = m0 + 16 * mi + 4 * r + lk;
       if (m >= g.M) continue;
       const double v = g.rscale[m % g.rsd] * acc[mi][r];
-      col[m] = g.accumulate ? col[m] + v : v;
+      if (split) unsafeAtomicAdd(col + m, v);
+      else col[m] = g.accumulate ? col[m] + v : v;
     }
 }
 
 static hipError_t launch_gemm_mfma(const GemmArgs& g, hipStream_t s) {
   if (g.ncol <= 0) return hipSuccess;
-  const int nz = g.zoff ? g.nz : 1;
+  const bool split = g.ksplit > 1;
+  if (split && (g.zoff || g.kchunk <= 0)) return hipErrorInvalidValue;
+  const int nz = split ? g.ksplit : (g.zoff ? g.nz : 1);
   dim3 grid(unsigned((g.M + GM_M - 1) / GM_M), unsigned((g.max_zcol + GM_N - 1) / GM_N), unsigned(nz));
   k_gemm_mfma<<<grid, dim3(256), 0, s>>>(g);
   return hipGetLastError();

@@ -40,6 +40,12 @@ SIGNATURES = {
     "kifmm_trg_dof": (C.c_int, [H]),
     "kifmm_evaluate": (C.c_int, [H, P_D, P_D]),
     "kifmm_evaluate_device": (C.c_int, [H, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kifmm_evaluate_multi": (C.c_int, [H, P_D, P_D, C.c_int]),
+    "kifmm_evaluate_multi_device": (C.c_int, [H, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "kifmm_evaluate_adjoint": (C.c_int, [H, P_D, P_D]),
+    "kifmm_field": (C.c_int, [H, P_D, P_D]),
+    "kifmm_field_device": (C.c_int, [H, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kifmm_evaluate_adjoint_device": (C.c_int, [H, C.c_void_p, C.c_void_p, C.c_void_p]),
     "kifmm_direct_device": (C.c_int, [H, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "kifmm_check": (C.c_int, [H, P_D, P_D, C.c_int, C.c_uint64, P_D]),
     "kifmm_tree_info": (C.c_int, [H, P_L]),
@@ -67,6 +73,13 @@ def lib():
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise KifmmError(-1, "native library not built: %s" % LIB_PATH)
+        # PyTorch ships its own HIP runtime.  Loading it before libkifmm makes the
+        # library bind to the runtime torch uses, so torch tensors, streams and
+        # the library share one runtime whichever module the caller imports first.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)
@@ -174,6 +187,39 @@ class FMM3d:
         self._chk(lib().kifmm_evaluate_device(self._h, C.c_void_p(d_src_den.data_ptr()),
                                               C.c_void_p(d_trg_val.data_ptr()),
                                               C.c_void_p(stream) if stream else None))
+
+    def evaluate_multi(self, src_den):
+        """nrhs density vectors (rows of src_den) -> nrhs value vectors."""
+        d = np.ascontiguousarray(src_den, dtype=np.float64).reshape(-1, self.n_src * self.src_dof)
+        out = np.zeros((d.shape[0], self.n_trg * self.trg_dof))
+        self._chk(lib().kifmm_evaluate_multi(self._h, _pd(d), _pd(out), d.shape[0]))
+        return out
+
+    def evaluate_adjoint(self, trg_in):
+        """Transposed pass A^T w: trgDOF per target in, srcDOF per source out."""
+        w = _f64(trg_in, self.n_trg * self.trg_dof)
+        out = np.zeros(self.n_src * self.src_dof)
+        self._chk(lib().kifmm_evaluate_adjoint(self._h, _pd(w), _pd(out)))
+        return out
+
+    def evaluate_adjoint_device(self, d_trg_in, d_src_out, stream=None):
+        """Transposed pass on device-resident float64 tensors."""
+        self._chk(lib().kifmm_evaluate_adjoint_device(self._h, C.c_void_p(d_trg_in.data_ptr()),
+                                                      C.c_void_p(d_src_out.data_ptr()),
+                                                      C.c_void_p(stream) if stream else None))
+
+    def field(self, src_den):
+        """Gradient of the potential at the targets, shape (n_trg, 3)."""
+        d = _f64(src_den, self.n_src * self.src_dof)
+        out = np.zeros(self.n_trg * 3)
+        self._chk(lib().kifmm_field(self._h, _pd(d), _pd(out)))
+        return out.reshape(-1, 3)
+
+    def field_device(self, d_src_den, d_trg_grad, stream=None):
+        """Gradient pass on device-resident float64 tensors (3 per target)."""
+        self._chk(lib().kifmm_field_device(self._h, C.c_void_p(d_src_den.data_ptr()),
+                                           C.c_void_p(d_trg_grad.data_ptr()),
+                                           C.c_void_p(stream) if stream else None))
 
     def direct_device(self, d_src_den, d_chk_pos, d_chk_val, stream=None):
         n = d_chk_pos.numel() // 3

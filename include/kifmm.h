@@ -77,6 +77,39 @@ int kifmm_evaluate(kifmm_handle h, const double *src_den, double *trg_val);
  * default stream); returns once enqueued. */
 int kifmm_evaluate_device(kifmm_handle h, const double *d_src_den, double *d_trg_val, void *stream);
 
+/* nrhs right-hand sides in one call (the nrhs argument proposed for
+ * KnlMat3d_MPI::evaluate, knlmat3d_mpi.hpp:62): src_den holds nrhs density
+ * vectors one after another (nrhs x n_src x srcDOF), trg_val receives nrhs value
+ * vectors (nrhs x n_trg x trgDOF), overwritten.  Host pointers, synchronous. */
+int kifmm_evaluate_multi(kifmm_handle h, const double *src_den, double *trg_val, int nrhs);
+
+/* Same with device pointers, enqueued on `stream`. */
+int kifmm_evaluate_multi_device(kifmm_handle h, const double *d_src_den, double *d_trg_val, int nrhs,
+                                void *stream);
+
+/* Transposed evaluate: src_out = A^T trg_in, where A is the operator that
+ * kifmm_evaluate applies (the transposed FMM the reference lacks: its
+ * evaluate, fmm3d_eval_mpi.cpp:396-1515, has no adjoint; every forward step is
+ * transposed in reverse order).  trg_in: trgDOF per target, src_out: srcDOF
+ * per source, caller's order, src_out overwritten.  Needs a symmetric
+ * equivalent kernel (kernels 111, 121, 312, 322, 511, 521); KIFMM_ERR_ARG
+ * otherwise.  Host pointers, synchronous. */
+int kifmm_evaluate_adjoint(kifmm_handle h, const double *trg_in, double *src_out);
+
+/* Same with device pointers, enqueued on `stream` (the first call on a handle
+ * builds the adjoint tables and synchronises `stream` first). */
+int kifmm_evaluate_adjoint_device(kifmm_handle h, const double *d_trg_in, double *d_src_out, void *stream);
+
+/* Field pass: the gradient of the evaluated potential at every target,
+ * 3 values per target in the caller's order (the reference computes values only;
+ * this is the position-gradient building block of gradient-based callers).
+ * Laplace single layer (kernel 111) only; KIFMM_ERR_ARG otherwise.  Host
+ * pointers, synchronous. */
+int kifmm_field(kifmm_handle h, const double *src_den, double *trg_grad);
+
+/* Same with device pointers, enqueued on `stream`. */
+int kifmm_field_device(kifmm_handle h, const double *d_src_den, double *d_trg_grad, void *stream);
+
 /* FP64 direct sum at nchk device-resident targets with the user kernel
  * (Dense3d_MPI::evaluate, dense3d_mpi.cpp:75-141). */
 int kifmm_direct_device(kifmm_handle h, const double *d_src_den, int nchk, const double *d_chk_pos,

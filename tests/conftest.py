@@ -17,5 +17,5 @@ def pytest_configure(config):
 def gpu():
     import torch
     if not torch.cuda.is_available():
-        pytest.skip("no GPU")
+        pytest.fail("GPU tests need a visible GPU (torch.cuda.is_available() is False)")
     return torch.device("cuda:0")

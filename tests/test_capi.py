@@ -102,3 +102,11 @@ def test_options_missing_file_and_option():
     with pytest.raises(kifmm.KifmmError) as e:
         kifmm.FMM3d.from_options(os.path.join(GOLDEN, "options_incomplete"))
     assert e.value.code == -4 and "let3d_maxLevel" in str(e.value)
+
+
+def test_adjoint_state_error():
+    f = kifmm.FMM3d(111, [1.0, 0.25], 4, 32, 30)
+    f.set_points(W.uniform(100, 1))
+    with pytest.raises(kifmm.KifmmError) as e:
+        f.evaluate_adjoint(np.zeros(100))
+    assert e.value.code == -3 and "before setup" in str(e.value)

@@ -1,0 +1,203 @@
+"""GPU: the transposed evaluate pass (kifmm_evaluate_adjoint) against the forward
+operator A of kifmm_evaluate.
+
+The reference has no adjoint, so the pass is pinned two ways:
+  * duality: <w, A sigma> = <A^T w, sigma> for random pairs of vectors, with A the
+    forward pass that the parity tests pin to the oracle; a wrong transpose, scale,
+    list inversion or level order in any step shows as a gap far above round-off;
+  * dense: on a small multi-level tree the matrix A is formed column by column from
+    the oracle's CPU evaluate (the restatement of fmm3d_eval_mpi.cpp:396-1515), and
+    A^T w is compared entry by entry.
+"""
+import numpy as np
+import pytest
+
+import kifmm
+import oracle as O
+from kifmm import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def dual_gap(f, seed):
+    sd, td = f.src_dof, f.trg_dof
+    rng = np.random.default_rng(seed)
+    sig = rng.standard_normal(f.n_src * sd)
+    w = rng.standard_normal(f.n_trg * td)
+    a = f.evaluate(sig)
+    at = f.evaluate_adjoint(w)
+    scale = np.linalg.norm(w) * np.linalg.norm(a) + np.linalg.norm(at) * np.linalg.norm(sig)
+    gap = abs(np.dot(w, a) - np.dot(at, sig)) / scale
+    print("duality gap %.3e" % gap, flush=True)
+    return gap
+
+
+ADJ_KERNELS = [(111, [1.0, 0.25], 4), (111, [1.0, 0.25], 6), (121, [1.0, 0.25], 4), (312, [1.0, 0.25], 4),
+               (322, [1.0, 0.25], 4), (511, [1.0, 0.3], 4), (521, [1.0, 0.3], 4)]
+
+
+@pytest.mark.parametrize("kt,coefs,np_", ADJ_KERNELS)
+def test_adjoint_duality_fp64(kt, coefs, np_):
+    """Adaptive sphere-surface points (U, V, W and X lists all active)."""
+    n = 6000
+    print("case adjoint kt=%d np=%d n=%d" % (kt, np_, n), flush=True)
+    pos, nor = W.spheres(n, 31 + kt, nsph=6)
+    f = kifmm.FMM3d(kt, coefs, np_, 24, 30)
+    f.set_points(pos, nor)
+    f.setup()
+    bar = 1e-12 if np_ == 4 else 1e-10  # np 6: the pseudo-inverse amplifies round-off
+    for seed in (1, 2):
+        assert dual_gap(f, seed) < bar
+
+
+def test_adjoint_duality_separate_targets():
+    """Sources and targets differ (the inverted lists are not the forward ones)."""
+    print("case adjoint separate targets", flush=True)
+    src = W.uniform(5000, 41)
+    trg = W.gaussians(4000, 42)
+    f = kifmm.FMM3d(111, [1.0, 0.25], 4, 20, 30)
+    f.set_points(src, None, trg)
+    f.setup()
+    assert dual_gap(f, 3) < 1e-12
+
+
+@pytest.mark.parametrize("kt,coefs", [(111, [1.0, 0.25]), (511, [1.0, 0.3])])
+def test_adjoint_duality_mixed(kt, coefs):
+    """Mixed precision: FP32 forward pieces bound the gap."""
+    n = 20000
+    print("case adjoint mixed kt=%d" % kt, flush=True)
+    pos = W.gaussians(n, 43)
+    f = kifmm.FMM3d(kt, coefs, 6, 32, 30, precision="mixed")
+    f.set_points(pos)
+    f.setup()
+    assert dual_gap(f, 4) < 1e-5
+
+
+def test_adjoint_dense_vs_oracle():
+    """A formed from the oracle's evaluate, one unit density at a time."""
+    n, np_, pm = 240, 4, 6
+    print("case adjoint dense n=%d" % n, flush=True)
+    pos = W.gaussians(n, 44)
+    nor = np.zeros_like(pos)
+    f = kifmm.FMM3d(111, [1.0, 0.25], np_, pm, 30)
+    f.set_points(pos, nor)
+    f.setup()
+    nue = np_ ** 3 - (np_ - 2) ** 3
+    nuc = (np_ + 2) ** 3 - np_ ** 3
+    pu = f.operator(0).reshape(nuc, nue).T
+    pd = f.operator(1).reshape(nue, nue).T
+    ops = O.Ops(111, [1.0, 0.25], np_, pu, pd)
+    tree = O.Tree(pos, None, pts_max=pm, max_level=30)
+    assert tree.lists["v"][1].size > 0, "the tree must have V lists"
+    A = np.empty((n, n))
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        A[:, j] = O.evaluate(tree, ops, 111, [1.0, 0.25], pos, nor, pos, e)
+    w = np.random.default_rng(5).standard_normal(n)
+    at = f.evaluate_adjoint(w)
+    ref = A.T @ w
+    r = np.linalg.norm(at - ref) / np.linalg.norm(ref)
+    print("relative difference %.3e" % r, flush=True)
+    assert r < 1e-12
+
+
+def direct_grad(pos, den, tgt):
+    """FP64 gradient of sum_s den_s / (4 pi |t - s|) at the rows of tgt (pairs
+    closer than 1e-8 excluded, as the reference kernel does)."""
+    d = tgt[:, None, :] - pos[None, :, :]
+    r2 = np.einsum("ijk,ijk->ij", d, d)
+    inv3 = np.where(r2 < 1e-16, 0.0, r2 ** -1.5)
+    return -np.einsum("ij,j,ijk->ik", inv3, den, d) / (4 * np.pi)
+
+
+@pytest.mark.parametrize("np_,precision,bar", [(6, "fp64", 3e-4), (8, "fp64", 3e-5), (6, "mixed", 3e-4)])
+def test_field_vs_direct(np_, precision, bar):
+    """Gradient pass (kifmm_field) against the FP64 direct gradient at 300 targets."""
+    n = 20000
+    print("case field np=%d %s" % (np_, precision), flush=True)
+    pos = W.gaussians(n, 45)
+    den = W.densities(n, 1, 46)
+    f = kifmm.FMM3d(111, [1.0, 0.25], np_, 40, 30, precision=precision)
+    f.set_points(pos)
+    f.setup()
+    g = f.field(den)
+    idx = np.random.default_rng(7).choice(n, 300, replace=False)
+    ref = direct_grad(pos, den, pos[idx])
+    r = np.linalg.norm(g[idx] - ref) / np.linalg.norm(ref)
+    print("relative error %.3e" % r, flush=True)
+    assert r < bar
+
+
+def test_field_rejects_other_kernels():
+    pos, nor = W.spheres(500, 47, nsph=2)
+    f = kifmm.FMM3d(321, [1.0, 0.25], 4, 32, 30)
+    f.set_points(pos, nor)
+    f.setup()
+    with pytest.raises(kifmm.KifmmError) as e:
+        f.field(np.zeros(500 * 3))
+    assert e.value.code == -2
+    with pytest.raises(kifmm.KifmmError) as e:
+        f.evaluate_adjoint(np.zeros(500 * 3))
+    assert e.value.code == -2
+
+
+def test_autograd_gradcheck():
+    """torch.autograd through fmm_apply: backward (the transposed pass) against
+    finite differences of the forward pass (exact for a linear map up to
+    rounding)."""
+    import torch
+    from kifmm.autograd import fmm_apply
+    n = 300
+    print("case autograd n=%d" % n, flush=True)
+    pos = W.gaussians(n, 48)
+    f = kifmm.FMM3d(111, [1.0, 0.25], 4, 8, 30)
+    f.set_points(pos)
+    f.setup()
+    den = torch.tensor(W.densities(n, 1, 49), dtype=torch.float64, device="cuda:0", requires_grad=True)
+    assert torch.autograd.gradcheck(lambda d: fmm_apply(f, d), (den,), eps=1e-6, atol=1e-7, rtol=1e-6)
+
+
+def test_autograd_loss_gradient():
+    """L = 1/2 |A sigma - A tau|^2: dL/dsigma from backward equals A^T (phi - psi)."""
+    import torch
+    from kifmm.autograd import fmm_apply
+    n = 20000
+    print("case autograd loss n=%d" % n, flush=True)
+    pos = W.uniform(n, 50)
+    f = kifmm.FMM3d(111, [1.0, 0.25], 6, 64, 30)
+    f.set_points(pos)
+    f.setup()
+    dev = torch.device("cuda:0")
+    sig = torch.tensor(W.densities(n, 1, 51), dtype=torch.float64, device=dev, requires_grad=True)
+    tau = torch.tensor(W.densities(n, 1, 52), dtype=torch.float64, device=dev)
+    phi = fmm_apply(f, sig)
+    psi = fmm_apply(f, tau)
+    loss = 0.5 * torch.sum((phi - psi) ** 2)
+    loss.backward()
+    ref = f.evaluate_adjoint((phi - psi).detach().cpu().numpy())
+    r = np.linalg.norm(sig.grad.cpu().numpy() - ref) / np.linalg.norm(ref)
+    print("relative difference %.3e" % r, flush=True)
+    assert r < 1e-13
+
+
+def test_position_grad_vs_direct():
+    """Position gradient of L = 1/2 |A sigma - A tau|^2 from two field passes
+    (kifmm.autograd.position_grad) against the same closed form on direct sums."""
+    from kifmm.autograd import position_grad
+    from test_gradient_formula import apply, field
+    n = 3000
+    print("case position gradient n=%d" % n, flush=True)
+    x = W.uniform(n, W.seed(3, 0))
+    sig = W.densities(n, 1, W.seed(3, 1))
+    tau = W.densities(n, 1, W.seed(3, 2))
+    f = kifmm.FMM3d(111, [1.0, 0.25], 6, 32, 30)
+    f.set_points(x)
+    f.setup()
+    u = sig - tau
+    pg = position_grad(f, u, f.evaluate(u))
+    g = apply(x, u)
+    ref = g[:, None] * field(x, u) + u[:, None] * field(x, g)
+    r = np.linalg.norm(pg - ref) / np.linalg.norm(ref)
+    print("relative error %.3e" % r, flush=True)
+    assert r < 1e-3

@@ -179,3 +179,28 @@ def test_mixed_precision(kt, np_, coefs, gen, tol):
     print("errors vs direct: fp64 %.3e mixed %.3e" % (out["fp64_err"], out["mixed_err"]), flush=True)
     assert d < tol
     assert out["mixed_err"] <= out["fp64_err"] + 2 * d
+
+
+def test_batched_problems_multi_rhs():
+    """The many-problem workload, reduced: independent Navier problems (np 8,
+    ptsMax 120, maxLevel 10, nu 0.3), each with 3 right-hand sides through
+    kifmm_evaluate_multi, every value vector against the oracle (FP64 bar)."""
+    rng = np.random.default_rng(W.seed(4, 0))
+    for p, n in enumerate((2048, 3500, 5000)):
+        pos = W.uniform(n, W.seed(4, 10 + p))
+        nor = np.zeros_like(pos)
+        den = rng.standard_normal((3, n * 3))
+        print("case batch problem %d n=%d nrhs=3" % (p, n), flush=True)
+        f = kifmm.FMM3d(511, [1.0, 0.3], 8, 120, 10)
+        f.set_points(pos, nor)
+        f.setup()
+        vals = f.evaluate_multi(den)
+        nue = 8 ** 3 - 6 ** 3
+        nuc = 10 ** 3 - 8 ** 3
+        pu = f.operator(0).reshape(nuc * 3, nue * 3).T
+        pd = f.operator(1).reshape(nue * 3, nue * 3).T
+        ops = O.Ops(511, [1.0, 0.3], 8, pu, pd)
+        tree = O.Tree(pos, None, pts_max=120, max_level=10)
+        for r in range(3):
+            ref = O.evaluate(tree, ops, 511, [1.0, 0.3], pos, nor, pos, den[r])
+            assert rel(vals[r], ref) <= PARITY
