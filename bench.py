@@ -43,6 +43,22 @@ def make_points(gen, n, cfg_index):
     raise ValueError(gen)
 
 
+def traffic_bytes(kernel):
+    """HBM bytes per launch of `kernel` on the 8M-point workload from the
+    committed rocprofv3 counter summary (profiles/hadamard_c3_traffic.json:
+    FETCH_SIZE and WRITE_SIZE in KiB per dispatch, scaled by the same counters
+    read on a 512 MiB device copy); None when no summary covers the kernel."""
+    path = os.path.join(ROOT, "profiles", "hadamard_c3_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        d = json.load(fh)
+    k = d.get("kernels", {}).get(kernel)
+    if not k:
+        return None
+    return int((k["fetch_kib"] * d["fetch_scale"] + k["write_kib"] * d["write_scale"]) * 1024)
+
+
 def cpu_baseline(cfg, precision_ops, n_sample, cfg_index):
     """The CPU oracle (reference algorithm restated in C, 1 thread) on a bounded
     sample of the same distribution; returns Mpoints/s and the sample size."""
@@ -148,15 +164,29 @@ def main():
     if dt64 is not None:
         diff_fp64 = float(np.linalg.norm(val - val64) / np.linalg.norm(val64))
 
-    # roofline of the dominant kernel (near field: U/W/X S2T, M2T, L2T)
+    # roofline of the dominant kernel: the V-list Hadamard (k_m2l_tile in the
+    # mixed path, k_hadamard in FP64), HBM-bound by its compulsory bytes: every
+    # source and target spectrum once and the 316 translation spectra,
+    # (n_vsrc*sd + n_vtrg*td)*H*2w + 316*sd*td*H*2w with w = 4 (FP32) or 8 (FP64).
     w = f.work_counts()
+    mixed = args.precision == "mixed"
+    had_ms = phases["VListHadamard"]
+    wbytes = 4 if mixed else 8
+    H = (np_ + 1) * (2 * np_) ** 2
+    sd = td = 1  # equivalent-density kernel 111
+    alg_bytes = (w["n_vsrc"] * sd + w["n_vtrg"] * td) * H * 2 * wbytes + 316 * sd * td * H * 2 * wbytes
+    achieved = alg_bytes / (had_ms * 1e-3) / 1e9
+    kname = "k_m2l_tile" if mixed else "k_hadamard"
+    roofline = {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_bytes(kname),
+                "kernel_ms": round(had_ms, 4), "algorithmic_bytes": alg_bytes}
+    # the near field (U/W/X S2T, M2T, L2T), the second-largest kernel: FP32/FP64
+    # vector bound, 11 flops per Laplace pair
     near_ms = phases["EvlUList"]
-    flops_per_pair = 11.0  # Laplace: 3 sub, 3 mul + 2 add (r^2), 1 rsqrt, 2 for the FMA
-    achieved = w["near_pairs"] * flops_per_pair / (near_ms * 1e-3) / 1e12
-    peak = PEAK_TFLOPS["fp64" if args.precision == "fp64" else "fp32"]
-    roofline = {"bound": "valu", "kernel": "k_near", "achieved": round(achieved, 3), "peak": peak,
-                "unit": "TFLOP/s", "frac": round(achieved / peak, 4), "traffic": None,
-                "kernel_ms": round(near_ms, 4)}
+    near_tf = w["near_pairs"] * 11.0 / (near_ms * 1e-3) / 1e12
+    peak = PEAK_TFLOPS["fp32" if mixed else "fp64"]
+    near = {"bound": "valu", "kernel": "k_near_lap_mx" if mixed else "k_near", "achieved": round(near_tf, 3),
+            "peak": peak, "unit": "TFLOP/s", "frac": round(near_tf / peak, 4), "kernel_ms": round(near_ms, 4)}
 
     cpu = None
     if not args.no_cpu:
@@ -191,6 +221,7 @@ def main():
         "rel_diff_vs_fp64": diff_fp64,
         "phases_ms": {k: round(v, 4) for k, v in phases.items()},
         "roofline": roofline,
+        "near_field": near,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -318,15 +318,21 @@ int kifmm_set_timing(kifmm_handle h, int on) {
   return KIFMM_OK;
 }
 
+int kifmm_set_graph(kifmm_handle h, int on) {
+  if (!h) return set_err(nullptr, KIFMM_ERR_ARG, "null handle");
+  h->fmm.enableGraph(on != 0);
+  return KIFMM_OK;
+}
+
 int kifmm_phase_times(kifmm_handle h, float* ms, int n) {
   if (!h || !ms) return set_err(h, KIFMM_ERR_ARG, "null argument");
   auto p = h->fmm.phaseMs();
-  for (int i = 0; i < n && i < kifmm::PH_COUNT; i++) ms[i] = p[i];
+  for (int i = 0; i < n && i < kifmm::PH_TIMES; i++) ms[i] = p[i];
   return KIFMM_OK;
 }
 
 const char* kifmm_phase_name(int phase) {
-  return (phase >= 0 && phase < kifmm::PH_COUNT) ? kifmm::kPhaseNames[phase] : "";
+  return (phase >= 0 && phase < kifmm::PH_TIMES) ? kifmm::kPhaseNames[phase] : "";
 }
 
 int kifmm_work_counts(kifmm_handle h, double* out, int n) {

@@ -120,7 +120,8 @@ hipError_t launch_near(int kt, const NearArgs& a, const KParams<double>& pu, con
                        hipStream_t s);
 hipError_t launch_surface(int kt, const SurfArgs& a, const KParams<double>& pu, hipStream_t s);
 hipError_t launch_gemm(const GemmArgs& g, hipStream_t s);
-hipError_t launch_vlist(const VlistArgs& v, hipStream_t s);
+// FFT, Hadamard, inverse FFT; e0/e1 (may be null) are recorded around the Hadamard.
+hipError_t launch_vlist(const VlistArgs& v, hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // Mixed-precision path (kernels_mixed.hip) and FP32-storage FFTs.
 hipError_t launch_near_mixed(int kt, const NearArgs& a, const KParams<double>& pu, const KParams<double>& pm,
                              hipStream_t s);

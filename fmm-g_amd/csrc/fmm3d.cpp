@@ -11,8 +11,8 @@
 
 namespace kifmm {
 
-const char* const kPhaseNames[PH_COUNT] = {"EvlIni", "EvlUpwComp", "EvlVList", "EvlXList",
-                                           "EvlCombine", "EvlUList", "EvlFinalize"};
+const char* const kPhaseNames[PH_TIMES] = {"EvlIni", "EvlUpwComp", "EvlVList", "EvlXList",
+                                           "EvlCombine", "EvlUList", "EvlFinalize", "VListHadamard"};
 
 DeviceBuffer::~DeviceBuffer() { release(); }
 
@@ -38,7 +38,18 @@ void DeviceBuffer::release() {
 }
 
 FMM3d::~FMM3d() {
+  drop_graph();
   for (auto ev : events_) (void)hipEventDestroy(ev);
+}
+
+void FMM3d::drop_graph() {
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  graph_exec_ = nullptr;
+  graph_ = nullptr;
+  graph_src_ = nullptr;
+  graph_dst_ = nullptr;
+  graph_stream_ = nullptr;
 }
 
 int FMM3d::srcDOF() const { return src_dof(kt_); }
@@ -67,6 +78,7 @@ int FMM3d::setPoints(int64_t ns, const double* spos, const double* snor, int64_t
 // setup: fmm3d_setup_mpi.cpp:41-565 (single rank)
 // ---------------------------------------------------------------------------
 int FMM3d::setup() {
+  drop_graph();
   int rc = setupHost();
   if (rc) return rc;
   rc = upload_state();
@@ -428,7 +440,7 @@ int FMM3d::upload_state() {
   }
 #undef KIFMM_UP
   if (events_.empty()) {
-    events_.resize(PH_COUNT + 1);
+    events_.resize(PH_COUNT + 3);  // phase boundaries, then the Hadamard's start and end
     for (auto& ev : events_)
       if ((rc = check(hipEventCreate(&ev), "create events"))) return rc;
   }
@@ -468,14 +480,19 @@ void FMM3d::vlist_args(VlistArgs& v, const VGroups& vg) const {
   v.m2l = m2l_d_.as<double2>(); v.dspec = dspec_d_.as<double2>(); v.espec = espec_d_.as<double2>();
 }
 
-int FMM3d::run_vlist(VlistArgs& v, hipStream_t s) {
+// timed: record the Hadamard's start and end events (forward pass with timing on).
+int FMM3d::run_vlist(VlistArgs& v, hipStream_t s, bool timed) {
   int rc;
   if (v.ngrp <= 0) return 0;
-  if (precision_ != PREC_MIXED) return check(launch_vlist(v, s), "V list");
+  hipEvent_t e0 = timed && timing_ ? events_[PH_COUNT + 1] : nullptr;
+  hipEvent_t e1 = timed && timing_ ? events_[PH_COUNT + 2] : nullptr;
+  if (precision_ != PREC_MIXED) return check(launch_vlist(v, s, e0, e1), "V list");
   if ((rc = check(launch_fft_fwd32(v, dspec32_d_.as<float2>(), s), "V list forward"))) return rc;
+  if (e0) (void)hipEventRecord(e0, s);
   if ((rc = check(launch_hadamard_mixed(v, dspec32_d_.as<float2>(), m2l32_d_.as<float2>(), espec32_d_.as<float2>(), s),
                   "V list Hadamard")))
     return rc;
+  if (e1) (void)hipEventRecord(e1, s);
   return check(launch_fft_inv32(v, espec32_d_.as<float2>(), s), "V list inverse");
 }
 
@@ -484,7 +501,29 @@ int FMM3d::run_vlist(VlistArgs& v, hipStream_t s) {
 // ---------------------------------------------------------------------------
 int FMM3d::evaluate(const double* d_src_den, double* d_trg_val, hipStream_t s) {
   if (!setup_done_) return fail(-3, "evaluate called before setup");
-  return evaluate_impl(d_src_den, d_trg_val, s, false);
+  if (!use_graph_ || !s || timing_) return evaluate_impl(d_src_den, d_trg_val, s, false);
+  if (!(graph_exec_ && graph_src_ == d_src_den && graph_dst_ == d_trg_val && graph_stream_ == s)) {
+    drop_graph();
+    int rc = check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin graph capture");
+    if (rc) return rc;
+    rc = evaluate_impl(d_src_den, d_trg_val, s, false);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if ((rc = check(e, "end graph capture"))) return rc;
+    graph_ = g;
+    if ((rc = check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "instantiate graph"))) {
+      drop_graph();
+      return rc;
+    }
+    graph_src_ = d_src_den;
+    graph_dst_ = d_trg_val;
+    graph_stream_ = s;
+  }
+  return check(hipGraphLaunch(graph_exec_, s), "graph launch");
 }
 
 int FMM3d::field(const double* d_src_den, double* d_trg_grad, hipStream_t s) {
@@ -576,7 +615,7 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
   vlist_args(v, vfwd_);
   v.UE = ue_d_.as<double>(); v.sz_ue = op.dat(UE);
   v.DC = dc_d_.as<double>(); v.sz_dc = op.dat(DC);
-  if ((rc = run_vlist(v, s))) return rc;
+  if ((rc = run_vlist(v, s, true))) return rc;
 
   // X list S2L (:1233-1265)
   mark(PH_XLIST);
@@ -623,6 +662,9 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
       (void)hipEventElapsedTime(&ms, events_[p], events_[p + 1]);
       phase_ms_[p] = ms;
     }
+    float ms = 0;
+    if (vfwd_.ngrp() > 0) (void)hipEventElapsedTime(&ms, events_[PH_COUNT + 1], events_[PH_COUNT + 2]);
+    phase_ms_[PH_HADAMARD] = ms;
   }
   return 0;
 }

@@ -22,7 +22,9 @@ enum Precision { PREC_FP64 = 0, PREC_MIXED = 1 };
 
 // Phase names follow the reference's PETSc events (manage_petsc_events.cpp:77-143).
 enum Phase { PH_INI = 0, PH_UPW, PH_VLIST, PH_XLIST, PH_DOWN, PH_NEAR, PH_FIN, PH_COUNT };
-extern const char* const kPhaseNames[PH_COUNT];
+// One more timed interval inside PH_VLIST: the Hadamard kernel alone.
+constexpr int PH_HADAMARD = PH_COUNT, PH_TIMES = PH_COUNT + 1;
+extern const char* const kPhaseNames[PH_TIMES];
 
 class DeviceBuffer {
  public:
@@ -100,7 +102,10 @@ class FMM3d {
   int64_t numTrg() const { return nt_; }
   const std::vector<double>& targetPositions() const { return tpos_; }
   void enableTiming(bool on) { timing_ = on; }
-  std::array<float, PH_COUNT> phaseMs() const { return phase_ms_; }
+  // Replay evaluate() from a captured hipGraph when it is called again with the
+  // same device buffers on the same non-default stream (timing off).
+  void enableGraph(bool on) { use_graph_ = on; if (!on) drop_graph(); }
+  std::array<float, PH_TIMES> phaseMs() const { return phase_ms_; }
   // Work counts for the roofline (pairs per phase), filled at setup.
   struct Work {
     double near_pairs = 0, near_exact_pairs = 0, s2m_pairs = 0, s2l_pairs = 0;
@@ -113,13 +118,14 @@ class FMM3d {
   int fail(int code, const std::string& msg) { err_ = msg; return code; }
   int check(hipError_t e, const char* what);
   int evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t s, bool grad);
+  void drop_graph();
   int build_tasks();
   int upload_state();
   int build_vgroups(const std::vector<int>& order, const std::vector<int64_t>& off, const std::vector<int>& lst,
                     bool need_evtr, VGroups& vg);
   int upload_vgroups(VGroups& vg);
   void vlist_args(VlistArgs& v, const VGroups& vg) const;
-  int run_vlist(VlistArgs& v, hipStream_t s);
+  int run_vlist(VlistArgs& v, hipStream_t s, bool timed);
   int build_adjoint();
 
   int kt_ = 111, np_ = 6, pts_max_ = 64, max_level_ = 30, root_level_ = 1, precision_ = PREC_FP64;
@@ -132,7 +138,7 @@ class FMM3d {
   Tree tree_;
   std::shared_ptr<const Operators> ops_;
   Work work_;
-  std::array<float, PH_COUNT> phase_ms_{};
+  std::array<float, PH_TIMES> phase_ms_{};
 
   // host task tables
   // One level-batched GEMM launch of the up or down pass.  mat: 0 = M2M
@@ -177,6 +183,12 @@ class FMM3d {
   DeviceBuffer direct_pos_d_, direct_nor_d_;
   DeviceBuffer m2l32_d_, dspec32_d_, espec32_d_;  // mixed path: FP32 spectra
   std::vector<hipEvent_t> events_;
+  bool use_graph_ = false;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t graph_exec_ = nullptr;
+  const double* graph_src_ = nullptr;
+  double* graph_dst_ = nullptr;
+  hipStream_t graph_stream_ = nullptr;
 };
 
 }  // namespace kifmm

@@ -26,6 +26,9 @@
 namespace kifmm {
 
 int FMM3d::build_adjoint() {
+  // the level tables and spectra buffers below are re-allocated: a captured
+  // evaluate graph would replay against the old ones
+  drop_graph();
   const Tree& t = tree_;
   const Operators& op = *ops_;
   const int N = t.n_nodes, sd = op.sd;
@@ -266,7 +269,7 @@ int FMM3d::evaluateAdjoint(const double* d_trg_in, double* d_src_out, hipStream_
   vlist_args(v, vadj_);
   v.UE = dc_d_.as<double>(); v.sz_ue = op.dat(DC);
   v.DC = ue_d_.as<double>(); v.sz_dc = op.dat(UE);
-  if ((rc = run_vlist(v, s))) return rc;
+  if ((rc = run_vlist(v, s, false))) return rc;
 
   if ((rc = run_steps(up_t_steps_))) return rc;
 

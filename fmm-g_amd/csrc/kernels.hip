@@ -489,11 +489,12 @@ static hipError_t launch_fft_inv_any(const VlistArgs& v, const C2* espec, hipStr
   return hipGetLastError();
 }
 
-hipError_t launch_vlist(const VlistArgs& v, hipStream_t s) {
+hipError_t launch_vlist(const VlistArgs& v, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   if (v.ngrp <= 0) return hipSuccess;
   if (v.np > kMaxNp) return hipErrorInvalidValue;
   hipError_t e = launch_fft_fwd_any<double2>(v, v.dspec, s);
   if (e != hipSuccess) return e;
+  if (e0) (void)hipEventRecord(e0, s);
   const double nrm = 1.0 / double(v.n * v.n * v.n);
   dim3 grid(unsigned(v.ngrp), unsigned((v.H + 63) / 64));
   if (v.sd == 1 && v.td == 1) k_hadamard<1, 1><<<grid, dim3(64), 0, s>>>(v, nrm);
@@ -502,6 +503,7 @@ hipError_t launch_vlist(const VlistArgs& v, hipStream_t s) {
   else return hipErrorInvalidValue;
   e = hipGetLastError();
   if (e != hipSuccess) return e;
+  if (e1) (void)hipEventRecord(e1, s);
   return launch_fft_inv_any<double2>(v, v.espec, s);
 }
 

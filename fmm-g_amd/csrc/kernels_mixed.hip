@@ -169,9 +169,10 @@ __global__ __launch_bounds__(kNearChunk) void k_near_mx(NearArgs a, KParams<floa
 // Laplace single layer (user and equivalent-density kernel 111): sources staged
 // as float4 (x, y, z relative to the leaf centre, density * 1/(4 pi)), one
 // broadcast LDS read per pair; each block is padded to kNearChunk entries
-// with zero weight so lanes sweep contiguous ranges unrolled by 4.  The source
-// split S is a power of two (lanes = S copies of the chunk's targets), and
-// the FP32 partial sums per item are added in FP64 as in k_near_mx.
+// with zero weight so lanes sweep contiguous ranges, unrolled by 4, of
+// ceil(kNearChunk / S) entries rounded up to a multiple of 4 (lanes = S copies
+// of the chunk's targets); the FP32 partial sums per item are added in FP64
+// as in k_near_mx.
 __global__ __launch_bounds__(kNearChunk) void k_near_lap_mx(NearArgs a, float pa_u, float pa_m) {
   __shared__ float4 s4[kNearChunk];
   __shared__ double red[kNearChunk * 2];
@@ -181,8 +182,7 @@ __global__ __launch_bounds__(kNearChunk) void k_near_lap_mx(NearArgs a, float pa
   const int tn = max(ch.tnum, 1);
   int S = kNearChunk / tn;
   S = S > 16 ? 16 : S;
-  S = 1 << (31 - __clz(S));
-  const int L = kNearChunk / S;  // multiple of 8
+  const int L = ((kNearChunk + S - 1) / S + 3) & ~3;
   const int ti = lane % tn, sub = lane / tn;
   const bool active = sub < S;
   float tx = 0, ty = 0, tz = 0;
@@ -384,15 +384,18 @@ __global__ __launch_bounds__(64) void k_hadamard_mx(VlistArgs v, const float2* _
 // frequency (64 per wave), so every index, halo lookup and branch is uniform
 // across the wave and runs on the scalar unit; for one colleague offset q the
 // 27 translation spectra 2q + [-1,1]^3 sit in registers and serve all kTileG
-// groups of the tile.  Absent halo sources (-1) are skipped; targets outside
-// the V-list target set are computed and dropped.  Frequency chunks are the
-// slow grid dimension and each XCD (workgroup id mod 8) sweeps a contiguous run
-// of tiles, so the spectra of neighbouring groups are re-read from its L2.
-constexpr int kTileG = 8;
+// groups of the tile.  The halo indices and spectra of the next (q, group) are
+// loaded while the current group's 64 multiply-adds run (the kernel is bound
+// by load latency at 2 waves per SIMD).  Absent halo sources (-1) are skipped;
+// targets outside the V-list target set are computed and dropped.  Frequency
+// chunks are the slow grid dimension and each XCD (workgroup id mod 8) sweeps a
+// contiguous run of tiles.
+constexpr int kTileG = 6;
 
 __global__ __launch_bounds__(64) void k_m2l_tile(VlistArgs v, const float2* __restrict__ dspec,
                                                  const float2* __restrict__ m2l, float2* __restrict__ espec,
                                                  float nrm, int ntile, int per_xcd) {
+  constexpr int G = kTileG;
   const int bx = blockIdx.x;
   const int tile = (bx & 7) * per_xcd + (bx >> 3);
   if (tile >= ntile) return;
@@ -400,13 +403,28 @@ __global__ __launch_bounds__(64) void k_m2l_tile(VlistArgs v, const float2* __re
   const int f = blockIdx.y * 64 + threadIdx.x;
   const bool fok = f < H;
   const int fs = fok ? f : H - 1;
-  const int g0 = tile * kTileG;
-  const int ng = min(kTileG, v.ngrp - g0);
-  float2 acc[kTileG][8];
+  const int g0 = tile * G;
+  const int ng = min(G, v.ngrp - g0);
+  float2 acc[G][8];
 #pragma unroll
-  for (int gi = 0; gi < kTileG; gi++)
+  for (int gi = 0; gi < G; gi++)
 #pragma unroll
     for (int t = 0; t < 8; t++) acc[gi][t] = make_float2(0.f, 0.f);
+  auto halo_base = [](int q) {
+    const int qx = q / 9 - 1, qy = (q / 3) % 3 - 1, qz = q % 3 - 1;
+    return (2 * qx + 2) * 36 + (2 * qy + 2) * 6 + (2 * qz + 2);
+  };
+  // halo source slots of group gi for colleague offset q, and their spectra
+  auto fetch = [&](int q, int gi, int* si, float2* x) {
+    const int* hal = v.halo + int64_t(g0 + (gi < ng ? gi : 0)) * 216 + halo_base(q);
+#pragma unroll
+    for (int c = 0; c < 8; c++) si[c] = gi < ng ? hal[(c >> 2) * 36 + ((c >> 1) & 1) * 6 + (c & 1)] : -1;
+#pragma unroll
+    for (int c = 0; c < 8; c++) x[c] = dspec[int64_t(si[c] < 0 ? 0 : si[c]) * H + fs];
+  };
+  int sc[8];
+  float2 xc[8];
+  fetch(0, 0, sc, xc);
   for (int q = 0; q < 27; q++) {
     if (q == 13) continue;  // the parent itself: siblings are adjacent
     const int qx = q / 9 - 1, qy = (q / 3) % 3 - 1, qz = q % 3 - 1;
@@ -417,34 +435,32 @@ __global__ __launch_bounds__(64) void k_m2l_tile(VlistArgs v, const float2* __re
       const int sl = sbase + (d / 9 - 1) * 49 + ((d / 3) % 3 - 1) * 7 + (d % 3 - 1);
       T[d] = m2l[int64_t(sl) * H + fs];
     }
-    const int hbase = (2 * qx + 2) * 36 + (2 * qy + 2) * 6 + (2 * qz + 2);
 #pragma unroll
-    for (int gi = 0; gi < kTileG; gi++) {
-      if (gi < ng) {
-        const int* hal = v.halo + int64_t(g0 + gi) * 216 + hbase;
-        int si[8];
+    for (int gi = 0; gi < G; gi++) {
+      int sn[8];
+      float2 xn[8];
+      const int qn = gi + 1 < G ? q : (q + 1 == 13 ? 14 : q + 1);
+      if (qn < 27) fetch(qn, gi + 1 < G ? gi + 1 : 0, sn, xn);
 #pragma unroll
-        for (int c = 0; c < 8; c++) si[c] = hal[(c >> 2) * 36 + ((c >> 1) & 1) * 6 + (c & 1)];
-        float2 x[8];
+      for (int c = 0; c < 8; c++) {
+        if (sc[c] < 0) continue;
 #pragma unroll
-        for (int c = 0; c < 8; c++) x[c] = dspec[int64_t(si[c] < 0 ? 0 : si[c]) * H + fs];
-#pragma unroll
-        for (int c = 0; c < 8; c++) {
-          if (si[c] < 0) continue;
-#pragma unroll
-          for (int t = 0; t < 8; t++) {
-            const int d = ((c >> 2) - (t >> 2) + 1) * 9 + (((c >> 1) & 1) - ((t >> 1) & 1) + 1) * 3 +
-                          ((c & 1) - (t & 1) + 1);
-            acc[gi][t].x = fmaf(T[d].x, x[c].x, fmaf(-T[d].y, x[c].y, acc[gi][t].x));
-            acc[gi][t].y = fmaf(T[d].x, x[c].y, fmaf(T[d].y, x[c].x, acc[gi][t].y));
-          }
+        for (int t = 0; t < 8; t++) {
+          const int d = ((c >> 2) - (t >> 2) + 1) * 9 + (((c >> 1) & 1) - ((t >> 1) & 1) + 1) * 3 +
+                        ((c & 1) - (t & 1) + 1);
+          acc[gi][t].x = fmaf(T[d].x, xc[c].x, fmaf(-T[d].y, xc[c].y, acc[gi][t].x));
+          acc[gi][t].y = fmaf(T[d].x, xc[c].y, fmaf(T[d].y, xc[c].x, acc[gi][t].y));
         }
+      }
+      if (qn < 27) {
+#pragma unroll
+        for (int c = 0; c < 8; c++) { sc[c] = sn[c]; xc[c] = xn[c]; }
       }
     }
   }
   if (!fok) return;
 #pragma unroll
-  for (int gi = 0; gi < kTileG; gi++) {
+  for (int gi = 0; gi < G; gi++) {
     if (gi >= ng) break;
 #pragma unroll
     for (int t = 0; t < 8; t++) {
@@ -508,8 +524,9 @@ hipError_t launch_hadamard_mixed(const VlistArgs& v, const float2* dspec, const 
 // Mixed path: FP64 matrix cores (v_mfma_f64_16x16x4f64, FMA, no order
 // contract).  160 x 64 tile per workgroup (M = 152 or 296 needs 1 or 2 row
 // tiles, 5% or 8% padding); wave w owns columns [16w, 16w+16) and all ten
-// 16-row blocks; K staged through LDS in slabs of 32 with the next slab
-// prefetched into registers while the current one feeds 80 MFMAs per wave.
+// 16-row blocks; K staged through LDS in slabs of 16 with the next slab
+// prefetched into registers while the current one feeds 40 MFMAs per wave
+// (168 VGPRs: 3 waves per SIMD).
 // With ksplit > 1, blockIdx.z picks a K chunk and the partial products are
 // added to C with FP64 atomics (the M2M of a level splits over the 8 children,
 // so a level with few parents still spreads over 8x the workgroups).  Operand
@@ -517,10 +534,10 @@ hipError_t launch_hadamard_mixed(const VlistArgs& v, const float2* dspec, const 
 // B[l/16][l%16], and its result register r holds D[4r + l/16][l%16] (rows
 // interleaved across the four lane groups).
 typedef double kd4 __attribute__((ext_vector_type(4)));
-constexpr int GM_M = 160, GM_N = 64, GM_K = 32, GM_MB = GM_M / 16;
+constexpr int GM_M = 160, GM_N = 64, GM_K = 16, GM_MB = GM_M / 16;
 constexpr int GM_RA = GM_M * GM_K / 256, GM_RB = GM_K * GM_N / 256;
 
-__global__ __launch_bounds__(256, 2) void k_gemm_mfma(GemmArgs g) {
+__global__ __launch_bounds__(256, 3) void k_gemm_mfma(GemmArgs g) {
   // row pitches = 16 mod 32 doubles: the four row groups of an operand read
   // fall on disjoint bank halves
   __shared__ double As[GM_K][GM_M + 16];
@@ -553,9 +570,9 @@ __global__ __launch_bounds__(256, 2) void k_gemm_mfma(GemmArgs g) {
       const int m = m0 + e % GM_M, k = k0 + e / GM_M;
       ra[u] = (m < g.M && k < kend) ? A[int64_t(k) * g.M + m] : 0.0;
     }
-    // B: lane owns column tid/4 and the 8 consecutive k from 8*(tid%4)
+    // B: lane owns column tid/4 and the GM_RB consecutive k from GM_RB*(tid%4)
     const int64_t col = bcol[tid >> 2];
-    const int kq0 = k0 + 8 * (tid & 3);
+    const int kq0 = k0 + GM_RB * (tid & 3);
 #pragma unroll
     for (int u = 0; u < GM_RB; u++) {
       const int kq = kq0 + u;
@@ -570,7 +587,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_mfma(GemmArgs g) {
       As[e / GM_M][e % GM_M] = ra[u];
     }
 #pragma unroll
-    for (int u = 0; u < GM_RB; u++) Bs[8 * (tid & 3) + u][tid >> 2] = rb[u];
+    for (int u = 0; u < GM_RB; u++) Bs[GM_RB * (tid & 3) + u][tid >> 2] = rb[u];
     __syncthreads();
     if (k0 + GM_K < kend) fetch(k0 + GM_K);
 #pragma unroll

@@ -23,7 +23,7 @@ P_L = C.POINTER(C.c_int64)
 H = C.c_void_p
 
 PREC = {"fp64": 0, "mixed": 1}
-PHASES = ["EvlIni", "EvlUpwComp", "EvlVList", "EvlXList", "EvlCombine", "EvlUList", "EvlFinalize"]
+PHASES = ["EvlIni", "EvlUpwComp", "EvlVList", "EvlXList", "EvlCombine", "EvlUList", "EvlFinalize", "VListHadamard"]
 WORK = ["near_pairs", "near_exact_pairs", "s2m_pairs", "s2l_pairs", "vlist_pairs", "n_vsrc",
         "n_vtrg", "n_vgrp", "up_cols", "m2m_cols", "dn_cols", "l2l_cols"]
 
@@ -42,6 +42,7 @@ SIGNATURES = {
     "kifmm_evaluate_device": (C.c_int, [H, C.c_void_p, C.c_void_p, C.c_void_p]),
     "kifmm_evaluate_multi": (C.c_int, [H, P_D, P_D, C.c_int]),
     "kifmm_evaluate_multi_device": (C.c_int, [H, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "kifmm_set_graph": (C.c_int, [H, C.c_int]),
     "kifmm_evaluate_adjoint": (C.c_int, [H, P_D, P_D]),
     "kifmm_field": (C.c_int, [H, P_D, P_D]),
     "kifmm_field_device": (C.c_int, [H, C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -262,6 +263,11 @@ class FMM3d:
         out = np.zeros(cnt.value)
         self._chk(lib().kifmm_get_operator(self._h, which, _pd(out), C.byref(cnt)))
         return out
+
+    def set_graph(self, on=True):
+        """Replay evaluate_device from a captured hipGraph (same buffers, same
+        non-default stream)."""
+        self._chk(lib().kifmm_set_graph(self._h, 1 if on else 0))
 
     def set_timing(self, on=True):
         self._chk(lib().kifmm_set_timing(self._h, 1 if on else 0))

@@ -137,9 +137,17 @@ int kifmm_get_list(kifmm_handle h, int kind, int64_t *off, int *lst);
 int kifmm_get_operator(kifmm_handle h, int which, double *out, int64_t *count);
 
 /* Per-phase device time of the last evaluate (PETSc event names of
- * manage_petsc_events.cpp:77-143) when timing is on; ms has KIFMM_NPHASE slots. */
-#define KIFMM_NPHASE 7
+ * manage_petsc_events.cpp:77-143) when timing is on; ms has KIFMM_NPHASE slots.
+ * The last slot, "VListHadamard", is the V-list Hadamard kernel alone, an
+ * interval inside "EvlVList". */
+#define KIFMM_NPHASE 8
 int kifmm_set_timing(kifmm_handle h, int on);
+/* Graph replay of kifmm_evaluate_device (no reference counterpart; for callers
+ * that evaluate repeatedly from the same device buffers): with on != 0, the
+ * first call with a given (d_src_den, d_trg_val, stream) on a non-default stream
+ * captures the whole pass into a hipGraph and later calls with the same triple
+ * replay it.  Other calls, the default stream and timing mode run eagerly. */
+int kifmm_set_graph(kifmm_handle h, int on);
 int kifmm_phase_times(kifmm_handle h, float *ms, int n);
 const char *kifmm_phase_name(int phase);
 /* Work counts from the built tree, used for roofline figures: near pairs, exact near

@@ -204,3 +204,37 @@ def test_batched_problems_multi_rhs():
         for r in range(3):
             ref = O.evaluate(tree, ops, 511, [1.0, 0.3], pos, nor, pos, den[r])
             assert rel(vals[r], ref) <= PARITY
+
+
+@pytest.mark.parametrize("precision,bar", [("fp64", 0.0), ("mixed", 1e-6)])
+def test_graph_replay(gpu, precision, bar):
+    """hipGraph replay (kifmm_set_graph) on a non-default stream: three replays
+    with new densities copied in between by eager torch work, each against an
+    eager evaluate of the same densities on a second handle.  Bitwise in FP64;
+    the mixed path sums the 8 children of an M2M with FP64 atomics in varying
+    order, which the np-6 pseudo-inverses amplify to ~3e-8 run to run."""
+    import torch
+    n = 20000
+    print("case graph replay %s" % precision, flush=True)
+    pos = W.gaussians(n, 61)
+    f = kifmm.FMM3d(111, [1.0, 0.25], 6, 64, 30, precision=precision)
+    f.set_points(pos)
+    f.setup()
+    g = kifmm.FMM3d(111, [1.0, 0.25], 6, 64, 30, precision=precision)
+    g.set_points(pos)
+    g.setup()
+    g.set_graph(True)
+    st = torch.cuda.Stream(device=gpu)
+    d_den = torch.empty(n, dtype=torch.float64, device=gpu)
+    d_val = torch.empty(n, dtype=torch.float64, device=gpu)
+    for rep in range(3):
+        den = W.densities(n, 1, 62 + rep)
+        with torch.cuda.stream(st):
+            d_den.copy_(torch.from_numpy(den).to(gpu))
+            g.evaluate_device(d_den, d_val, stream=st.cuda_stream)
+        st.synchronize()
+        got = d_val.cpu().numpy()
+        ref = f.evaluate(den)
+        r = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+        print("replay %d relative difference %.3e" % (rep, r), flush=True)
+        assert r <= bar
