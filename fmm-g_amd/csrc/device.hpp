@@ -56,8 +56,8 @@ struct GemmArgs {
   // several matrices in one launch: matrix z is A + z*a_zstride and owns the
   // columns [zoff[z], zoff[z+1]); zoff == nullptr means one matrix, all columns
   const int* zoff; int nz; int64_t a_zstride; int max_zcol;
-  // split-K (mixed path only, zoff == nullptr): workgroup z sums k in
-  // [z*kchunk, (z+1)*kchunk) and adds its partial product to C atomically
+  // split-K (mixed path only): blockIdx.z = matrix * ksplit + c sums k in
+  // [c*kchunk, (c+1)*kchunk) and adds its partial product to C atomically
   int ksplit; int kchunk;
 };
 

@@ -499,6 +499,25 @@ int FMM3d::run_vlist(VlistArgs& v, hipStream_t s, bool timed) {
 // ---------------------------------------------------------------------------
 // evaluate: fmm3d_eval_mpi.cpp:396-1515
 // ---------------------------------------------------------------------------
+void split_k(GemmArgs& g) {
+  // A launch's time is its tile count over the CUs times the serial K loop
+  // (~3 us per 16-deep slab).  Splitting K pays where that loop is long and the
+  // tiles few: the top levels (<= 16 tiles, any K) and the M2M of levels with
+  // few parents (K = 8 nUE >= 1024, <= 160 tiles).  Elsewhere the FP64 atomic
+  // adds of the partial products cost more than the split saves (measured).
+  const int64_t base = int64_t((g.M + 159) / 160) * ((g.max_zcol + 63) / 64) * (g.zoff ? g.nz : 1);
+  int64_t want = 0, cap = 16;
+  if (base <= 16) want = 128;
+  else if (g.K >= 1024 && base <= 160) { want = 512; cap = 8; }
+  if (want == 0 || g.K <= 16) return;
+  int ks = int(std::min<int64_t>({cap, (want + base - 1) / base, g.K / 16}));
+  if (ks <= 1) return;
+  int chunk = ((g.K + ks - 1) / ks + 15) / 16 * 16;
+  g.kchunk = chunk;
+  g.ksplit = (g.K + chunk - 1) / chunk;
+  if (g.ksplit <= 1) { g.ksplit = 0; g.kchunk = 0; }
+}
+
 int FMM3d::evaluate(const double* d_src_den, double* d_trg_val, hipStream_t s) {
   if (!setup_done_) return fail(-3, "evaluate called before setup");
   if (!use_graph_ || !s || timing_) return evaluate_impl(d_src_den, d_trg_val, s, false);
@@ -592,9 +611,10 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
       if (st.rs_off >= 0) { g.rscale = scales_d_.as<double>() + st.rs_off; g.rsd = op.sd; }
       else { g.rscale = ones_d_.as<double>(); g.rsd = 1; }
       g.ncol = st.ncol; g.accumulate = st.accumulate;
-      // mixed path: the M2M sum over the 8 children runs as 8 K chunks whose
-      // partial products are added atomically into the zeroed parent UC
-      if (mixed && st.mat == 0) { g.ksplit = 8; g.kchunk = op.dat(UE); }
+      // mixed path: launches with very few workgroups split K into chunks of
+      // whole 16-deep slabs; the partial products are added atomically into
+      // outputs that start at zero (pinv, M2M) or accumulate (L2L)
+      if (mixed) split_k(g);
       int r = check(mixed ? launch_gemm_fma(g, s) : launch_gemm(g, s), "level GEMM");
       if (r) return r;
     }

@@ -54,6 +54,9 @@ struct VGroups {
   int ngrp() const { return int(tgt.size() / 8); }
 };
 
+// Split K of a mixed-path level GEMM launch that would have too few workgroups.
+void split_k(GemmArgs& g);
+
 class FMM3d {
  public:
   FMM3d() = default;

@@ -245,7 +245,6 @@ int FMM3d::evaluateAdjoint(const double* d_trg_in, double* d_src_out, hipStream_
         case 6: g.M = op.dat(DC); g.K = op.dat(DE); g.A = pdnT_d_.as<double>(); break;
         default:
           g.M = op.dat(DE); g.K = 8 * op.dat(DC); g.A = l2lT_d_.as<double>();
-          g.ksplit = 8; g.kchunk = op.dat(DC);  // sum over the 8 children in parallel
       }
       g.max_zcol = stp.ncol;
       g.B = bufs[stp.src]; g.ldb = szs[stp.src];
@@ -257,6 +256,9 @@ int FMM3d::evaluateAdjoint(const double* d_trg_in, double* d_src_out, hipStream_
       if (stp.rs_off >= 0) { g.rscale = scales_d_.as<double>() + stp.rs_off; g.rsd = op.sd; }
       else { g.rscale = ones_d_.as<double>(); g.rsd = 1; }
       g.ncol = stp.ncol; g.accumulate = 1;
+      // every adjoint step accumulates, so K chunks may add atomically; the FP64
+      // mode keeps one workgroup per output tile and is deterministic
+      if (precision_ == PREC_MIXED) split_k(g);
       int r = check(launch_gemm_fma(g, s), "adjoint level GEMM");
       if (r) return r;
     }

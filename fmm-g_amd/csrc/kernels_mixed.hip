@@ -527,9 +527,10 @@ hipError_t launch_hadamard_mixed(const VlistArgs& v, const float2* dspec, const 
 // 16-row blocks; K staged through LDS in slabs of 16 with the next slab
 // prefetched into registers while the current one feeds 40 MFMAs per wave
 // (168 VGPRs: 3 waves per SIMD).
-// With ksplit > 1, blockIdx.z picks a K chunk and the partial products are
-// added to C with FP64 atomics (the M2M of a level splits over the 8 children,
-// so a level with few parents still spreads over 8x the workgroups).  Operand
+// With ksplit > 1, blockIdx.z = matrix * ksplit + K chunk and the partial
+// products are added to C with FP64 atomics, so a level with few columns still
+// spreads over enough workgroups (C must then hold its starting value: zero
+// for the pinv and M2M outputs, which are written once).  Operand
 // layout of the 16x16x4 instruction: lane l supplies A[l%16][l/16] and
 // B[l/16][l%16], and its result register r holds D[4r + l/16][l%16] (rows
 // interleaved across the four lane groups).
@@ -543,14 +544,16 @@ __global__ __launch_bounds__(256, 3) void k_gemm_mfma(GemmArgs g) {
   __shared__ double As[GM_K][GM_M + 16];
   __shared__ double Bs[GM_K][GM_N + 16];
   __shared__ int64_t bcol[GM_N];
-  const int z = blockIdx.z;
   const bool split = g.ksplit > 1;
-  const int cbeg = (!split && g.zoff) ? g.zoff[z] : 0, cend = (!split && g.zoff) ? g.zoff[z + 1] : g.ncol;
+  const int zk = split ? int(blockIdx.z) % g.ksplit : 0;  // K chunk
+  const int z = split ? int(blockIdx.z) / g.ksplit : int(blockIdx.z);  // matrix
+  const int cbeg = g.zoff ? g.zoff[z] : 0, cend = g.zoff ? g.zoff[z + 1] : g.ncol;
   const int n0 = cbeg + int(blockIdx.y) * GM_N;
   if (n0 >= cend) return;
-  const double* A = g.A + (split ? int64_t(0) : int64_t(z) * g.a_zstride);
-  const int kbeg = split ? z * g.kchunk : 0;
+  const double* A = g.A + int64_t(z) * g.a_zstride;
+  const int kbeg = zk * g.kchunk;
   const int kend = split ? min(g.K, kbeg + g.kchunk) : g.K;
+  if (kbeg >= kend) return;
   const int m0 = blockIdx.x * GM_M;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
   if (tid < GM_N) {
@@ -617,8 +620,8 @@ __global__ __launch_bounds__(256, 3) void k_gemm_mfma(GemmArgs g) {
 static hipError_t launch_gemm_mfma(const GemmArgs& g, hipStream_t s) {
   if (g.ncol <= 0) return hipSuccess;
   const bool split = g.ksplit > 1;
-  if (split && (g.zoff || g.kchunk <= 0)) return hipErrorInvalidValue;
-  const int nz = split ? g.ksplit : (g.zoff ? g.nz : 1);
+  if (split && (g.kchunk <= 0 || int64_t(g.ksplit) * g.kchunk < g.K)) return hipErrorInvalidValue;
+  const int nz = (g.zoff ? g.nz : 1) * (split ? g.ksplit : 1);
   dim3 grid(unsigned((g.M + GM_M - 1) / GM_M), unsigned((g.max_zcol + GM_N - 1) / GM_N), unsigned(nz));
   k_gemm_mfma<<<grid, dim3(256), 0, s>>>(g);
   return hipGetLastError();
