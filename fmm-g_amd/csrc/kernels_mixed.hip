@@ -166,13 +166,33 @@ __global__ __launch_bounds__(kNearChunk) void k_near_mx(NearArgs a, KParams<floa
   }
 }
 
-// Laplace single layer (user and equivalent-density kernel 111): sources staged
-// as float4 (x, y, z relative to the leaf centre, density * 1/(4 pi)), one
-// broadcast LDS read per pair; each block is padded to kNearChunk entries
-// with zero weight so lanes sweep contiguous ranges, unrolled by 4, of
-// ceil(kNearChunk / S) entries rounded up to a multiple of 4 (lanes = S copies
-// of the chunk's targets); the FP32 partial sums per item are added in FP64
-// as in k_near_mx.
+// Laplace single layer (user and equivalent-density kernel 111): the sources of
+// an item are staged as float4 (x, y, z relative to the leaf centre, density /
+// 4 pi), one broadcast LDS read per pair.  Lanes form S copies of the chunk's
+// targets; each copy sweeps a contiguous share of the staged block (its source
+// count rounded up to 4, split S ways in multiples of 4), unrolled by 4.
+// Padding entries have zero weight and lie 1e18 away, so they add exact zeros.
+// Pairs closer than the reference's 1e-8 cut contribute nothing.  (Skipping
+// that test where the host can prove no such pair exists was measured slower.)
+// The FP32 partial sums per item are added in FP64, near (exact, UE) and far
+// (DE) parts separately.
+__device__ __forceinline__ void lap_sweep(const float4* s4, int j0, int j1, float tx, float ty, float tz,
+                                          float& acc0, float& acc1) {
+  const float eps2 = float(kMinDif * kMinDif);
+  for (int j = j0; j < j1; j += 4) {
+#pragma unroll
+    for (int u = 0; u < 4; u += 2) {
+      const float4 s0 = s4[j + u], s1 = s4[j + u + 1];
+      const float dx0 = tx - s0.x, dy0 = ty - s0.y, dz0 = tz - s0.z;
+      const float dx1 = tx - s1.x, dy1 = ty - s1.y, dz1 = tz - s1.z;
+      const float r0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
+      const float r1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
+      acc0 = fmaf(r0 < eps2 ? 0.f : rsqrtf(r0), s0.w, acc0);
+      acc1 = fmaf(r1 < eps2 ? 0.f : rsqrtf(r1), s1.w, acc1);
+    }
+  }
+}
+
 __global__ __launch_bounds__(kNearChunk) void k_near_lap_mx(NearArgs a, float pa_u, float pa_m) {
   __shared__ float4 s4[kNearChunk];
   __shared__ double red[kNearChunk * 2];
@@ -182,7 +202,6 @@ __global__ __launch_bounds__(kNearChunk) void k_near_lap_mx(NearArgs a, float pa
   const int tn = max(ch.tnum, 1);
   int S = kNearChunk / tn;
   S = S > 16 ? 16 : S;
-  const int L = ((kNearChunk + S - 1) / S + 3) & ~3;
   const int ti = lane % tn, sub = lane / tn;
   const bool active = sub < S;
   float tx = 0, ty = 0, tz = 0;
@@ -190,7 +209,6 @@ __global__ __launch_bounds__(kNearChunk) void k_near_lap_mx(NearArgs a, float pa
     const double* p = a.tpos + 3 * int64_t(ch.tbeg + ti);
     tx = float(p[0] - cb.x); ty = float(p[1] - cb.y); tz = float(p[2] - cb.z);
   }
-  const float eps2 = float(kMinDif * kMinDif);
   double near = 0.0, far = 0.0;
   for (int it = ch.ibeg; it < ch.iend; it++) {
     const int code = a.items[it];
@@ -213,7 +231,7 @@ __global__ __launch_bounds__(kNearChunk) void k_near_lap_mx(NearArgs a, float pa
     for (int base = 0; base < ns; base += kNearChunk) {
       const int m = min(kNearChunk, ns - base);
       __syncthreads();
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      float4 v = make_float4(1e18f, 0.f, 0.f, 0.f);
       if (lane < m) {
         if (exact) {
           const int64_t q = int64_t(b + base + lane);
@@ -228,19 +246,10 @@ __global__ __launch_bounds__(kNearChunk) void k_near_lap_mx(NearArgs a, float pa
       s4[lane] = v;
       __syncthreads();
       if (active) {
-        const int j0 = sub * L, j1 = min(j0 + L, (m + 3) & ~3);
-        for (int j = j0; j < j1; j += 4) {
-#pragma unroll
-          for (int u = 0; u < 4; u += 2) {
-            const float4 s0 = s4[j + u], s1 = s4[j + u + 1];
-            const float dx0 = tx - s0.x, dy0 = ty - s0.y, dz0 = tz - s0.z;
-            const float dx1 = tx - s1.x, dy1 = ty - s1.y, dz1 = tz - s1.z;
-            const float r0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
-            const float r1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
-            acc0 = fmaf(r0 < eps2 ? 0.f : rsqrtf(r0), s0.w, acc0);
-            acc1 = fmaf(r1 < eps2 ? 0.f : rsqrtf(r1), s1.w, acc1);
-          }
-        }
+        const int m4 = (m + 3) & ~3;
+        const int L = ((m4 + S - 1) / S + 3) & ~3;  // this block's share per copy
+        const int j0 = sub * L, j1 = min(j0 + L, m4);
+        lap_sweep(s4, j0, j1, tx, ty, tz, acc0, acc1);
       }
     }
     if (kind == ITEM_DE) far += double(acc0 + acc1);
