@@ -196,17 +196,26 @@ int FMM3d::build_tasks() {
   const int rl = t.root_level;
   std::vector<std::vector<int>> by_depth(t.max_depth + 1);
   for (int i = 0; i < N; i++) by_depth[t.depth[i]].push_back(i);
+  const bool fuse_up = precision_ == PREC_MIXED;
   for (int d = t.max_depth; d >= 1; d--) {  // :557-595, children before parents
-    std::vector<int> mb, mc, pb;
+    std::vector<int> mb, mc, pb, lb;
     for (int i : by_depth[d]) {
       if (!(t.tag[i] & TAG_CBTR)) continue;
       pb.push_back(i);
       if (!t.leaf(i)) { mb.push_back(t.child[i]); mc.push_back(i); }
+      else lb.push_back(i);
     }
     work_.m2m_cols += double(mc.size());
     work_.up_cols += double(pb.size());
-    add_step(up_steps_, 0, 0, 1, mb, mc, scale_vec(d + 1 + rl, +1), -1, 0);
-    add_step(up_steps_, 1, 1, 0, pb, pb, -1, scale_vec(d + rl, -1), 0);
+    if (fuse_up) {
+      // mixed path: a parent's UE straight from its children's UE through
+      // W = P_up [M_0 ... M_7] (UC is formed for the leaves only)
+      add_step(up_steps_, 8, 0, 0, mb, mc, scale_vec(d + 1 + rl, +1), scale_vec(d + rl, -1), 0);
+      add_step(up_steps_, 1, 1, 0, lb, lb, -1, scale_vec(d + rl, -1), 0);
+    } else {
+      add_step(up_steps_, 0, 0, 1, mb, mc, scale_vec(d + 1 + rl, +1), -1, 0);
+      add_step(up_steps_, 1, 1, 0, pb, pb, -1, scale_vec(d + rl, -1), 0);
+    }
   }
   for (int d = 2; d <= t.max_depth; d++) {  // :1368-1382, parents before children
     std::vector<int> pb;
@@ -420,6 +429,20 @@ int FMM3d::upload_state() {
   KIFMM_UP(pup_d_, op.pinv_up, "upload pinv up");
   KIFMM_UP(pdn_d_, op.pinv_dn, "upload pinv down");
   KIFMM_UP(m2m_d_, op.m2m, "upload M2M");
+  if (precision_ == PREC_MIXED) {
+    // W = P_up M (nUE x 8 nUE, column-major), formed once in FP64
+    const int ue = op.dat(UE), uc = op.dat(UC);
+    std::vector<double> w(size_t(ue) * 8 * ue, 0.0);
+#pragma omp parallel for
+    for (int k = 0; k < 8 * ue; k++)
+      for (int j = 0; j < uc; j++) {
+        const double mjk = op.m2m[size_t(k) * uc + j];
+        for (int m = 0; m < ue; m++) w[size_t(k) * ue + m] += op.pinv_up[size_t(j) * ue + m] * mjk;
+      }
+    KIFMM_UP(wup_d_, w, "upload P_up M2M");
+  } else {
+    wup_d_.release();
+  }
   KIFMM_UP(l2l_d_, op.l2l, "upload L2L");
   KIFMM_UP(m2l_d_, op.m2l, "upload M2L");
   // V list
@@ -590,6 +613,7 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
       switch (st.mat) {
         case 0: g.M = op.dat(UC); g.K = 8 * op.dat(UE); g.A = m2m_d_.as<double>(); break;
         case 1: g.M = op.dat(UE); g.K = op.dat(UC); g.A = pup_d_.as<double>(); break;
+        case 8: g.M = op.dat(UE); g.K = 8 * op.dat(UE); g.A = wup_d_.as<double>(); break;
         case 2: g.M = op.dat(DE); g.K = op.dat(DC); g.A = pdn_d_.as<double>(); break;
         default:
           g.M = op.dat(DC); g.K = op.dat(DE);

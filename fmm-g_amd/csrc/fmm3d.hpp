@@ -148,7 +148,8 @@ class FMM3d {
   // (the 8 child matrices side by side), 1 = UC->UE pinv, 2 = DC->DE pinv,
   // 3 = L2L (the 8 child matrices, one per z slice); transposed tables of the
   // adjoint: 4 = M2M^T (stacked, 8*UE x UC), 5 = pinv_up^T, 6 = pinv_dn^T,
-  // 7 = L2L^T (side by side, DE x 8*DC).  src/dst: 0 = UE, 1 = UC, 2 = DE, 3 = DC.
+  // 7 = L2L^T (side by side, DE x 8*DC); mixed up pass: 8 = P_up [M_0 ... M_7]
+  // (UE of the children -> UE of the parent).  src/dst: 0 = UE, 1 = UC, 2 = DE, 3 = DC.
   struct GemmStep {
     int mat, src, dst, ncol, accumulate;
     int64_t idx_off;  // bidx at idx_off, cidx at idx_off + ncol
@@ -183,6 +184,7 @@ class FMM3d {
   DeviceBuffer dspec_d_, espec_d_, tgrad_d_;
   DeviceBuffer nt_chunks_d_, nt_items_d_, ue_t_tasks_d_, de_t_tasks_d_, surf_t_items_d_, tbeg_d_, tnum_d_;
   DeviceBuffer m2mT_d_, pupT_d_, pdnT_d_, l2lT_d_;
+  DeviceBuffer wup_d_;  // mixed path: P_up [M_0 ... M_7]
   DeviceBuffer direct_pos_d_, direct_nor_d_;
   DeviceBuffer m2l32_d_, dspec32_d_, espec32_d_;  // mixed path: FP32 spectra
   std::vector<hipEvent_t> events_;
