@@ -409,17 +409,45 @@ __global__ __launch_bounds__(64) void k_fft_inv(VlistArgs v, const C2* espec) {
   for (int i = 0; i < v.td; i++) {
     const C2* in = espec + (int64_t(slot) * v.td + i) * (N * N * HH);
     __syncthreads();
-    // along z (a), only block rows a' in [0,np); line (kb, kc)
-    for (int e = lane; e < N * HH; e += 64) {
-      double2 x[N];
+    // along z (a), only block rows a' in [0,np); line (kb, kc).  When the n*h
+    // lines fit two rounds of the 64 lanes (np <= 7), both rounds' inputs are
+    // loaded before either is transformed, so the load latency is paid once
+    // per node.
+    if constexpr (N * HH > 128) {
+      for (int e = lane; e < N * HH; e += 64) {
+        double2 x[N];
 #pragma unroll
-      for (int ka = 0; ka < N; ka++) x[ka] = load_c(in + ka * N * HH + e);
+        for (int ka = 0; ka < N; ka++) x[ka] = load_c(in + ka * N * HH + e);
 #pragma unroll
-      for (int a = 0; a < NP; a++) {
-        double2 acc = make_double2(0, 0);
+        for (int a = 0; a < NP; a++) {
+          double2 acc = make_double2(0, 0);
 #pragma unroll
-        for (int ka = 0; ka < N; ka++) acc = cadd(acc, cmul(x[ka], w[((a + O) * ka) % N]));
-        y1[a * N * HH + e] = acc;
+          for (int ka = 0; ka < N; ka++) acc = cadd(acc, cmul(x[ka], w[((a + O) * ka) % N]));
+          y1[a * N * HH + e] = acc;
+        }
+      }
+    } else {
+      const int e0 = lane, e1 = lane + 64;
+      const bool has1 = e1 < N * HH;
+      C2 r0[N], r1[N];
+#pragma unroll
+      for (int ka = 0; ka < N; ka++) r0[ka] = in[ka * N * HH + (e0 < N * HH ? e0 : 0)];
+#pragma unroll
+      for (int ka = 0; ka < N; ka++) r1[ka] = in[ka * N * HH + (has1 ? e1 : 0)];
+#pragma unroll
+      for (int half = 0; half < 2; half++) {
+        const int e = half ? e1 : e0;
+        if (e >= N * HH) continue;
+        double2 x[N];
+#pragma unroll
+        for (int ka = 0; ka < N; ka++) x[ka] = load_c(half ? &r1[ka] : &r0[ka]);
+#pragma unroll
+        for (int a = 0; a < NP; a++) {
+          double2 acc = make_double2(0, 0);
+#pragma unroll
+          for (int ka = 0; ka < N; ka++) acc = cadd(acc, cmul(x[ka], w[((a + O) * ka) % N]));
+          y1[a * N * HH + e] = acc;
+        }
       }
     }
     __syncthreads();
