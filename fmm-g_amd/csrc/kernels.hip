@@ -276,12 +276,25 @@ __global__ __launch_bounds__(64) void k_fft_fwd(VlistArgs v, C2* dspec) {
 #pragma unroll
   for (int m = 0; m < N; m++) w[m] = v.twiddle[m];
   const int l = v.depth[node] + v.root_level;
+  constexpr int R = (NSAM + 63) / 64;  // lane rounds over the surface samples
   for (int s = 0; s < v.sd; s++) {
     const double scale = ldexp(1.0, int(l * v.deg[s]));
+    // all rounds' samples are loaded before the grid is cleared and filled
+    double uv[R];
+    int sh[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int q = lane + 64 * r;
+      const bool ok = q < NSAM;
+      sh[r] = ok ? v.shell[q] : 0;
+      uv[r] = ok ? v.UE[int64_t(node) * v.sz_ue + q * v.sd + s] * scale : 0.0;
+    }
     __syncthreads();
     for (int e = lane; e < NP * NP * NP; e += 64) xb[e] = 0.0;
     __syncthreads();
-    for (int q = lane; q < NSAM; q += 64) xb[v.shell[q]] = v.UE[int64_t(node) * v.sz_ue + q * v.sd + s] * scale;
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      if (lane + 64 * r < NSAM) xb[sh[r]] = uv[r];
     __syncthreads();
     // along x (c): real -> half spectrum; line (a, b)
     for (int ab = lane; ab < NP * NP; ab += 64) {
