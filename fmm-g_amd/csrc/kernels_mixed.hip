@@ -166,12 +166,17 @@ __global__ __launch_bounds__(kNearChunk) void k_near_mx(NearArgs a, KParams<floa
   }
 }
 
-// Laplace single layer (user and equivalent-density kernel 111): the sources of
-// an item are staged as float4 (x, y, z relative to the leaf centre, density /
-// 4 pi), one broadcast LDS read per pair.  Lanes form S copies of the chunk's
-// targets; each copy sweeps a contiguous share of the staged block (its source
-// count rounded up to 4, split S ways in multiples of 4), unrolled by 4.
-// Padding entries have zero weight and lie 1e18 away, so they add exact zeros.
+// Laplace single layer (user and equivalent-density kernel 111).  The sources of
+// a chunk's items are packed, item after item, into blocks of kNearChunk float4
+// (x, y, z relative to the leaf centre, density / 4 pi): a lane finds its item
+// by binary search over an LDS prefix of the item sizes (a wave-shuffle scan),
+// so the ~50 small items of a leaf (37 points on average) share a few staging
+// rounds.  Exact and UE items are packed together (near part), the DE item (last
+// in the list) separately (far part).  One broadcast LDS read per pair; lanes
+// form S copies of the chunk's targets and each copy sweeps a contiguous share
+// of the block (source count rounded up to 4, split S ways in multiples of 4),
+// unrolled by 4.  Padding entries have zero weight and lie 1e18 away, so they
+// add exact zeros.
 // Pairs closer than the reference's 1e-8 cut contribute nothing.  (Skipping
 // that test where the host can prove no such pair exists was measured slower.)
 // The FP32 partial sums per item are added in FP64, near (exact, UE) and far
@@ -193,8 +198,13 @@ __device__ __forceinline__ void lap_sweep(const float4* s4, int j0, int j1, floa
   }
 }
 
+constexpr int kItemWin = 128;
 __global__ __launch_bounds__(kNearChunk) void k_near_lap_mx(NearArgs a, float pa_u, float pa_m) {
   __shared__ float4 s4[kNearChunk];
+  __shared__ int pre[kItemWin + 1];
+  __shared__ int ibeg_[kItemWin];
+  __shared__ unsigned char ikind_[kItemWin];
+  __shared__ int wsum[2];
   __shared__ double red[kNearChunk * 2];
   const NearChunk ch = a.chunks[blockIdx.x];
   const double4 cb = a.geom[ch.node];
@@ -209,59 +219,81 @@ __global__ __launch_bounds__(kNearChunk) void k_near_lap_mx(NearArgs a, float pa
     const double* p = a.tpos + 3 * int64_t(ch.tbeg + ti);
     tx = float(p[0] - cb.x); ty = float(p[1] - cb.y); tz = float(p[2] - cb.z);
   }
-  double near = 0.0, far = 0.0;
-  for (int it = ch.ibeg; it < ch.iend; it++) {
-    const int code = a.items[it];
-    const int node = code >> 2, kind = code & 3;
-    const bool exact = kind == ITEM_EXACT;
-    int ns, b = 0;
-    double4 g = make_double4(0, 0, 0, 0);
-    const double* sam = nullptr;
-    const double* den = nullptr;
-    if (exact) {
-      b = a.src_beg[node]; ns = a.src_num[node];
-    } else {
-      g = a.geom[node];
-      const bool ue = kind == ITEM_UE;
-      sam = ue ? a.sam_ue : a.sam_de;
-      ns = ue ? a.n_ue : a.n_de;
-      den = ue ? a.UE + int64_t(node) * a.sz_ue : a.DE + int64_t(node) * a.sz_de;
-    }
-    float acc0 = 0.f, acc1 = 0.f;
-    for (int base = 0; base < ns; base += kNearChunk) {
-      const int m = min(kNearChunk, ns - base);
+  const int idiv = ch.iend > ch.ibeg && (a.items[ch.iend - 1] & 3) == ITEM_DE ? ch.iend - 1 : ch.iend;
+  double part[2] = {0.0, 0.0};
+  for (int pass = 0; pass < 2; pass++) {
+    const int i0 = pass ? idiv : ch.ibeg, i1 = pass ? ch.iend : idiv;
+    for (int w0 = i0; w0 < i1; w0 += kItemWin) {
+      const int nw = min(i1 - w0, kItemWin);
       __syncthreads();
-      float4 v = make_float4(1e18f, 0.f, 0.f, 0.f);
-      if (lane < m) {
-        if (exact) {
-          const int64_t q = int64_t(b + base + lane);
-          v = make_float4(float(a.spos[3 * q] - cb.x), float(a.spos[3 * q + 1] - cb.y),
-                          float(a.spos[3 * q + 2] - cb.z), float(a.sden[q]) * pa_u);
-        } else {
-          const int q = base + lane;
-          v = make_float4(float((g.x - cb.x) + g.w * sam[3 * q]), float((g.y - cb.y) + g.w * sam[3 * q + 1]),
-                          float((g.z - cb.z) + g.w * sam[3 * q + 2]), float(den[q]) * pa_m);
+      int sz = 0;
+      if (lane < nw) {
+        const int code = a.items[w0 + lane], node = code >> 2, kind = code & 3;
+        const bool ex = kind == ITEM_EXACT;
+        sz = ex ? a.src_num[node] : (kind == ITEM_UE ? a.n_ue : a.n_de);
+        ibeg_[lane] = ex ? a.src_beg[node] : node;
+        ikind_[lane] = (unsigned char)kind;
+      }
+      int inc = sz;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(inc, d, 64);
+        if ((lane & 63) >= d) inc += y;
+      }
+      if ((lane & 63) == 63) wsum[lane >> 6] = inc;
+      __syncthreads();
+      if (lane >= 64) inc += wsum[0];
+      pre[lane + 1] = inc;
+      if (lane == 0) pre[0] = 0;
+      __syncthreads();
+      const int total = pre[nw];
+      for (int base = 0; base < total; base += kNearChunk) {
+        const int m = min(kNearChunk, total - base);
+        __syncthreads();
+        float4 v = make_float4(1e18f, 0.f, 0.f, 0.f);
+        if (lane < m) {
+          const int p = base + lane;
+          int lo = 0, hi = nw - 1;
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (pre[mid] <= p) lo = mid;
+            else hi = mid - 1;
+          }
+          const int off = p - pre[lo], kind = ikind_[lo];
+          if (kind == ITEM_EXACT) {
+            const int64_t q = int64_t(ibeg_[lo] + off);
+            v = make_float4(float(a.spos[3 * q] - cb.x), float(a.spos[3 * q + 1] - cb.y),
+                            float(a.spos[3 * q + 2] - cb.z), float(a.sden[q]) * pa_u);
+          } else {
+            const int node = ibeg_[lo];
+            const double4 g = a.geom[node];
+            const double* sam = kind == ITEM_UE ? a.sam_ue : a.sam_de;
+            const double* den = kind == ITEM_UE ? a.UE + int64_t(node) * a.sz_ue : a.DE + int64_t(node) * a.sz_de;
+            v = make_float4(float((g.x - cb.x) + g.w * sam[3 * off]), float((g.y - cb.y) + g.w * sam[3 * off + 1]),
+                            float((g.z - cb.z) + g.w * sam[3 * off + 2]), float(den[off]) * pa_m);
+          }
+        }
+        s4[lane] = v;
+        __syncthreads();
+        if (active) {
+          const int m4 = (m + 3) & ~3;
+          const int L = ((m4 + S - 1) / S + 3) & ~3;
+          const int j0 = sub * L, j1 = min(j0 + L, m4);
+          float acc0 = 0.f, acc1 = 0.f;
+          lap_sweep(s4, j0, j1, tx, ty, tz, acc0, acc1);
+          part[pass] += double(acc0 + acc1);
         }
       }
-      s4[lane] = v;
-      __syncthreads();
-      if (active) {
-        const int m4 = (m + 3) & ~3;
-        const int L = ((m4 + S - 1) / S + 3) & ~3;  // this block's share per copy
-        const int j0 = sub * L, j1 = min(j0 + L, m4);
-        lap_sweep(s4, j0, j1, tx, ty, tz, acc0, acc1);
-      }
     }
-    if (kind == ITEM_DE) far += double(acc0 + acc1);
-    else near += double(acc0 + acc1);
   }
   __syncthreads();
   if (active && sub > 0) {
-    red[(lane - tn) * 2] = near;
-    red[(lane - tn) * 2 + 1] = far;
+    red[(lane - tn) * 2] = part[0];
+    red[(lane - tn) * 2 + 1] = part[1];
   }
   __syncthreads();
   if (lane < ch.tnum) {
+    double near = part[0], far = part[1];
     for (int u = 1; u < S; u++) {
       near += red[((u - 1) * tn + lane) * 2];
       far += red[((u - 1) * tn + lane) * 2 + 1];
