@@ -165,9 +165,13 @@ def main():
         diff_fp64 = float(np.linalg.norm(val - val64) / np.linalg.norm(val64))
 
     # roofline of the dominant kernel: the V-list Hadamard (k_m2l_tile in the
-    # mixed path, k_hadamard in FP64), HBM-bound by its compulsory bytes: every
-    # source and target spectrum once and the 316 translation spectra,
-    # (n_vsrc*sd + n_vtrg*td)*H*2w + 316*sd*td*H*2w with w = 4 (FP32) or 8 (FP64).
+    # mixed path, k_hadamard in FP64).  Algorithmic work: one complex
+    # multiply-add (8 flops) per V-list (target, source) pair, frequency and
+    # sd*td component.  Compulsory bytes: every source and target spectrum once
+    # and the 316 translation spectra, (n_vsrc*sd + n_vtrg*td)*H*2w +
+    # 316*sd*td*H*2w with w = 4 (FP32) or 8 (FP64).  The intensity (about 78
+    # flop/byte in FP32) is far above the machine balance (157.3 TFLOP/s over
+    # 8 TB/s, about 20), so the compute peak is the bound.
     w = f.work_counts()
     mixed = args.precision == "mixed"
     had_ms = phases["VListHadamard"]
@@ -175,11 +179,15 @@ def main():
     H = (np_ + 1) * (2 * np_) ** 2
     sd = td = 1  # equivalent-density kernel 111
     alg_bytes = (w["n_vsrc"] * sd + w["n_vtrg"] * td) * H * 2 * wbytes + 316 * sd * td * H * 2 * wbytes
-    achieved = alg_bytes / (had_ms * 1e-3) / 1e9
+    alg_flops = 8.0 * w["vlist_pairs"] * H * sd * td
+    had_tf = alg_flops / (had_ms * 1e-3) / 1e12
+    had_peak = PEAK_TFLOPS["fp32" if mixed else "fp64"]
     kname = "k_m2l_tile" if mixed else "k_hadamard"
-    roofline = {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_bytes(kname),
-                "kernel_ms": round(had_ms, 4), "algorithmic_bytes": alg_bytes}
+    roofline = {"bound": "mfma", "kernel": kname, "achieved": round(had_tf, 3), "peak": had_peak,
+                "unit": "TFLOP/s", "frac": round(had_tf / had_peak, 4), "traffic": traffic_bytes(kname),
+                "kernel_ms": round(had_ms, 4), "algorithmic_flops": alg_flops, "algorithmic_bytes": alg_bytes,
+                "intensity_flop_per_byte": round(alg_flops / alg_bytes, 2),
+                "hbm_gbs_at_algorithmic_bytes": round(alg_bytes / (had_ms * 1e-3) / 1e9, 1)}
     # the near field (U/W/X S2T, M2T, L2T), the second-largest kernel: FP32/FP64
     # vector bound, 11 flops per Laplace pair
     near_ms = phases["EvlUList"]

@@ -423,15 +423,25 @@ __global__ __launch_bounds__(64) void k_hadamard_mx(VlistArgs v, const float2* _
 // translation slot is 2q + (c - tau), and the 27 slots with |offset| <= 1 hold
 // zeros, so the whole halo can be swept with no per-pair lists.  Lane =
 // frequency (64 per wave), so every index, halo lookup and branch is uniform
-// across the wave and runs on the scalar unit; for one colleague offset q the
-// 27 translation spectra 2q + [-1,1]^3 sit in registers and serve all kTileG
-// groups of the tile.  The halo indices and spectra of the next (q, group) are
-// loaded while the current group's 64 multiply-adds run (the kernel is bound
-// by load latency at 2 waves per SIMD).  Absent halo sources (-1) are skipped;
-// targets outside the V-list target set are computed and dropped.  Frequency
-// chunks are the slow grid dimension and each XCD (workgroup id mod 8) sweeps a
-// contiguous run of tiles.
-constexpr int kTileG = 6;
+// across the wave; for one colleague offset q the 27 translation spectra
+// 2q + [-1,1]^3 sit in registers and serve all kTileG groups of the tile.
+//
+// * The halo rows of the tile's groups (216 source slots each) are staged in
+//   LDS once, so a step's eight source indices come from LDS (a short, fixed
+//   latency) instead of scalar loads that miss the scalar cache on a table of
+//   ngrp * 864 bytes.
+// * The source spectra of the next (q, group) are loaded while the current
+//   group's multiply-adds run.
+// * Three real accumulators per complex output (Karatsuba).  With T = a + ib
+//   and X = c + id: K1 += c (a + b), K2 += a (d - c), K3 += b (c + d), so
+//   Re = K1 - K3 = sum (ac - bd) and Im = K1 + K2 = sum (ad + bc).  a + b is
+//   formed once per translation spectrum and q, d - c and c + d once per source
+//   spectrum and step: 3 FMAs per complex multiply-add instead of 4.
+//
+// Absent halo sources (-1) are skipped; targets outside the V-list target set
+// are computed and dropped.  Frequency chunks are the slow grid dimension and
+// each XCD (workgroup id mod 8) sweeps a contiguous run of tiles.
+constexpr int kTileG = 5;
 
 __global__ __launch_bounds__(64) void k_m2l_tile(VlistArgs v, const float2* __restrict__ dspec,
                                                  const float2* __restrict__ m2l, float2* __restrict__ espec,
@@ -446,20 +456,25 @@ __global__ __launch_bounds__(64) void k_m2l_tile(VlistArgs v, const float2* __re
   const int fs = fok ? f : H - 1;
   const int g0 = tile * G;
   const int ng = min(G, v.ngrp - g0);
-  float2 acc[G][8];
+  __shared__ int shal[G * 216];
+  for (int i = threadIdx.x; i < G * 216; i += 64) shal[i] = i < ng * 216 ? v.halo[int64_t(g0) * 216 + i] : -1;
+  __syncthreads();
+  float k1[G][8], k2[G][8], k3[G][8];
 #pragma unroll
   for (int gi = 0; gi < G; gi++)
 #pragma unroll
-    for (int t = 0; t < 8; t++) acc[gi][t] = make_float2(0.f, 0.f);
+    for (int t = 0; t < 8; t++) k1[gi][t] = k2[gi][t] = k3[gi][t] = 0.f;
   auto halo_base = [](int q) {
     const int qx = q / 9 - 1, qy = (q / 3) % 3 - 1, qz = q % 3 - 1;
     return (2 * qx + 2) * 36 + (2 * qy + 2) * 6 + (2 * qz + 2);
   };
-  // halo source slots of group gi for colleague offset q, and their spectra
+  // halo source slots of group gi for colleague offset q (wave-uniform), and
+  // their spectra
   auto fetch = [&](int q, int gi, int* si, float2* x) {
-    const int* hal = v.halo + int64_t(g0 + (gi < ng ? gi : 0)) * 216 + halo_base(q);
+    const int* hal = shal + gi * 216 + halo_base(q);
 #pragma unroll
-    for (int c = 0; c < 8; c++) si[c] = gi < ng ? hal[(c >> 2) * 36 + ((c >> 1) & 1) * 6 + (c & 1)] : -1;
+    for (int c = 0; c < 8; c++)
+      si[c] = __builtin_amdgcn_readfirstlane(hal[(c >> 2) * 36 + ((c >> 1) & 1) * 6 + (c & 1)]);
 #pragma unroll
     for (int c = 0; c < 8; c++) x[c] = dspec[int64_t(si[c] < 0 ? 0 : si[c]) * H + fs];
   };
@@ -470,11 +485,14 @@ __global__ __launch_bounds__(64) void k_m2l_tile(VlistArgs v, const float2* __re
     if (q == 13) continue;  // the parent itself: siblings are adjacent
     const int qx = q / 9 - 1, qy = (q / 3) % 3 - 1, qz = q % 3 - 1;
     const int sbase = (2 * qx + 3) * 49 + (2 * qy + 3) * 7 + (2 * qz + 3);
-    float2 T[27];
+    float ta[27], tb[27], ts[27];
 #pragma unroll
     for (int d = 0; d < 27; d++) {
       const int sl = sbase + (d / 9 - 1) * 49 + ((d / 3) % 3 - 1) * 7 + (d % 3 - 1);
-      T[d] = m2l[int64_t(sl) * H + fs];
+      const float2 t = m2l[int64_t(sl) * H + fs];
+      ta[d] = t.x;
+      tb[d] = t.y;
+      ts[d] = t.x + t.y;
     }
 #pragma unroll
     for (int gi = 0; gi < G; gi++) {
@@ -485,12 +503,14 @@ __global__ __launch_bounds__(64) void k_m2l_tile(VlistArgs v, const float2* __re
 #pragma unroll
       for (int c = 0; c < 8; c++) {
         if (sc[c] < 0) continue;
+        const float xr = xc[c].x, xm = xc[c].y - xc[c].x, xs = xc[c].x + xc[c].y;
 #pragma unroll
         for (int t = 0; t < 8; t++) {
           const int d = ((c >> 2) - (t >> 2) + 1) * 9 + (((c >> 1) & 1) - ((t >> 1) & 1) + 1) * 3 +
                         ((c & 1) - (t & 1) + 1);
-          acc[gi][t].x = fmaf(T[d].x, xc[c].x, fmaf(-T[d].y, xc[c].y, acc[gi][t].x));
-          acc[gi][t].y = fmaf(T[d].x, xc[c].y, fmaf(T[d].y, xc[c].x, acc[gi][t].y));
+          k1[gi][t] = fmaf(xr, ts[d], k1[gi][t]);
+          k2[gi][t] = fmaf(xm, ta[d], k2[gi][t]);
+          k3[gi][t] = fmaf(xs, tb[d], k3[gi][t]);
         }
       }
       if (qn < 27) {
@@ -506,7 +526,8 @@ __global__ __launch_bounds__(64) void k_m2l_tile(VlistArgs v, const float2* __re
 #pragma unroll
     for (int t = 0; t < 8; t++) {
       const int tg = v.tgt[(g0 + gi) * 8 + t];
-      if (tg >= 0) espec[int64_t(tg) * H + f] = make_float2(acc[gi][t].x * nrm, acc[gi][t].y * nrm);
+      if (tg >= 0)
+        espec[int64_t(tg) * H + f] = make_float2((k1[gi][t] - k3[gi][t]) * nrm, (k1[gi][t] + k2[gi][t]) * nrm);
     }
   }
 }
