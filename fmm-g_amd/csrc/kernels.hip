@@ -10,6 +10,8 @@
 //   write back (:1490-1510)                               k_scatter
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "device.hpp"
 #include "gemm.hpp"
 
@@ -255,6 +257,65 @@ __device__ inline double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 __device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+
+// 3- and 6-point DFTs, X[k] = sum_n x[n] exp(SGN 2 pi i n k / N) (SGN = -1:
+// forward, +1: inverse).  Three points: t1 = b + c, t2 = a - t1/2,
+// r = SGN i (sqrt(3)/2)(b - c); X = (a + t1, t2 + r, t2 - r).  Six points: radix 2
+// over two 3-point transforms of the even and odd samples with twiddles
+// w6^k = exp(SGN 2 pi i k / 6), X[k] = E[k] + w6^k D[k], X[k+3] = E[k] - w6^k D[k].
+template <int SGN>
+__device__ inline void dft3(double2 a, double2 b, double2 c, double2& X0, double2& X1, double2& X2) {
+  constexpr double kS = 0.86602540378443864676;
+  const double2 t1 = cadd(b, c);
+  const double2 t2 = make_double2(a.x - 0.5 * t1.x, a.y - 0.5 * t1.y);
+  const double2 d = csub(b, c);
+  const double2 r = make_double2(-SGN * kS * d.y, SGN * kS * d.x);
+  X0 = cadd(a, t1);
+  X1 = cadd(t2, r);
+  X2 = csub(t2, r);
+}
+template <int SGN>
+__device__ inline void dft6(const double2 (&x)[6], double2 (&X)[6]) {
+  constexpr double kS = 0.86602540378443864676;
+  double2 E[3], D[3];
+  dft3<SGN>(x[0], x[2], x[4], E[0], E[1], E[2]);
+  dft3<SGN>(x[1], x[3], x[5], D[0], D[1], D[2]);
+  const double2 t1 = cmul(make_double2(0.5, SGN * kS), D[1]);
+  const double2 t2 = cmul(make_double2(-0.5, SGN * kS), D[2]);
+  X[0] = cadd(E[0], D[0]); X[3] = csub(E[0], D[0]);
+  X[1] = cadd(E[1], t1);   X[4] = csub(E[1], t1);
+  X[2] = cadd(E[2], t2);   X[5] = csub(E[2], t2);
+}
+// Mixed path, np = 6.  Forward line transform from the 6 block samples at grid
+// positions b + 3 to the 12 frequencies: out[k] = sum_b in[b] w^((b+3) k); the
+// even outputs are the 6-point DFT of the samples placed at (b + 3) mod 6, the
+// odd outputs the same of in[b] w^(b+3).  w: the 12 forward twiddles.
+__device__ inline void pad12_fwd(const double2 (&in)[6], const double2 (&w)[12], double2 (&out)[12]) {
+  double2 e[6], o[6], E[6], D[6];
+#pragma unroll
+  for (int b = 0; b < 6; b++) {
+    const int p = (b + 3) % 6;
+    e[p] = in[b];
+    o[p] = cmul(in[b], w[(b + 3) % 12]);
+  }
+  dft6<-1>(e, E);
+  dft6<-1>(o, D);
+#pragma unroll
+  for (int m = 0; m < 6; m++) { out[2 * m] = E[m]; out[2 * m + 1] = D[m]; }
+}
+// Inverse line transform from 12 frequencies to the 6 block positions a + 3:
+// out[a] = sum_k in[k] wi^((a+3) k) = E[j] + wi^(a+3) D[j], j = (a + 3) mod 6,
+// with E, D the 6-point inverse DFTs of the even and odd frequencies.
+__device__ inline void pad12_inv(const double2 (&in)[12], const double2 (&wi)[12], double2 (&out)[6]) {
+  double2 e[6], o[6], E[6], D[6];
+#pragma unroll
+  for (int m = 0; m < 6; m++) { e[m] = in[2 * m]; o[m] = in[2 * m + 1]; }
+  dft6<1>(e, E);
+  dft6<1>(o, D);
+#pragma unroll
+  for (int a = 0; a < 6; a++) out[a] = cadd(E[(a + 3) % 6], cmul(wi[(a + 3) % 12], D[(a + 3) % 6]));
+}
 
 __device__ inline void store_c(double2* p, double2 v) { *p = v; }
 __device__ inline void store_c(float2* p, double2 v) { *p = make_float2(float(v.x), float(v.y)); }
@@ -265,10 +326,16 @@ __device__ inline double2 load_c(const float2* p) { const float2 v = *p; return 
 template <int NP, typename C2>
 __global__ __launch_bounds__(64) void k_fft_fwd(VlistArgs v, C2* dspec) {
   constexpr int N = 2 * NP, HH = NP + 1, O = NP / 2;
+  // mixed path at np = 6: the y and z line transforms by two 6-point DFTs
+  // (the FP64 path keeps the ascending-order sums of the reference layout)
+  constexpr bool kFast = NP == 6 && std::is_same<C2, float2>::value;
   constexpr int NSAM = NP * NP * NP - (NP - 2) * (NP - 2) * (NP - 2);
+  // mixed path: the line results between passes are kept in LDS as FP32 (the
+  // spectra are stored in FP32 anyway), which halves the LDS of a workgroup
+  using LT = typename std::conditional<std::is_same<C2, float2>::value, float2, double2>::type;
   __shared__ double xb[NP * NP * NP];
-  __shared__ double2 t1[NP * NP * HH];
-  __shared__ double2 t2[NP * N * HH];
+  __shared__ LT t1[NP * NP * HH];
+  __shared__ LT t2[NP * N * HH];
   const int slot = blockIdx.x;
   const int node = v.src_node[slot];
   const int lane = threadIdx.x;
@@ -309,7 +376,7 @@ __global__ __launch_bounds__(64) void k_fft_fwd(VlistArgs v, C2* dspec) {
           const double2 tw = w[((c + O) * kc) % N];
           acc.x += xv[c] * tw.x; acc.y += xv[c] * tw.y;
         }
-        t1[ab * HH + kc] = acc;
+        store_c(&t1[ab * HH + kc], acc);
       }
     }
     __syncthreads();
@@ -318,13 +385,20 @@ __global__ __launch_bounds__(64) void k_fft_fwd(VlistArgs v, C2* dspec) {
       const int a = e / HH, kc = e % HH;
       double2 in[NP];
 #pragma unroll
-      for (int b = 0; b < NP; b++) in[b] = t1[(a * NP + b) * HH + kc];
+      for (int b = 0; b < NP; b++) in[b] = load_c(&t1[(a * NP + b) * HH + kc]);
+      if constexpr (kFast) {
+        double2 out[N];
+        pad12_fwd(in, w, out);
 #pragma unroll
-      for (int kb = 0; kb < N; kb++) {
-        double2 acc = make_double2(0, 0);
+        for (int kb = 0; kb < N; kb++) store_c(&t2[(a * N + kb) * HH + kc], out[kb]);
+      } else {
 #pragma unroll
-        for (int b = 0; b < NP; b++) acc = cadd(acc, cmul(in[b], w[((b + O) * kb) % N]));
-        t2[(a * N + kb) * HH + kc] = acc;
+        for (int kb = 0; kb < N; kb++) {
+          double2 acc = make_double2(0, 0);
+#pragma unroll
+          for (int b = 0; b < NP; b++) acc = cadd(acc, cmul(in[b], w[((b + O) * kb) % N]));
+          store_c(&t2[(a * N + kb) * HH + kc], acc);
+        }
       }
     }
     __syncthreads();
@@ -333,13 +407,20 @@ __global__ __launch_bounds__(64) void k_fft_fwd(VlistArgs v, C2* dspec) {
     for (int e = lane; e < N * HH; e += 64) {
       double2 in[NP];
 #pragma unroll
-      for (int a = 0; a < NP; a++) in[a] = t2[a * N * HH + e];
+      for (int a = 0; a < NP; a++) in[a] = load_c(&t2[a * N * HH + e]);
+      if constexpr (kFast) {
+        double2 o12[N];
+        pad12_fwd(in, w, o12);
 #pragma unroll
-      for (int ka = 0; ka < N; ka++) {
-        double2 acc = make_double2(0, 0);
+        for (int ka = 0; ka < N; ka++) store_c(out + ka * N * HH + e, o12[ka]);
+      } else {
 #pragma unroll
-        for (int a = 0; a < NP; a++) acc = cadd(acc, cmul(in[a], w[((a + O) * ka) % N]));
-        store_c(out + ka * N * HH + e, acc);
+        for (int ka = 0; ka < N; ka++) {
+          double2 acc = make_double2(0, 0);
+#pragma unroll
+          for (int a = 0; a < NP; a++) acc = cadd(acc, cmul(in[a], w[((a + O) * ka) % N]));
+          store_c(out + ka * N * HH + e, acc);
+        }
       }
     }
   }
@@ -405,9 +486,11 @@ __global__ __launch_bounds__(64) void k_hadamard(VlistArgs v, double nrm) {
 template <int NP, typename C2>
 __global__ __launch_bounds__(64) void k_fft_inv(VlistArgs v, const C2* espec) {
   constexpr int N = 2 * NP, HH = NP + 1, O = NP / 2;
+  constexpr bool kFast = NP == 6 && std::is_same<C2, float2>::value;
   constexpr int NSAM = NP * NP * NP - (NP - 2) * (NP - 2) * (NP - 2);
-  __shared__ double2 y1[NP * N * HH];
-  __shared__ double2 y2[NP * NP * HH];
+  using LT = typename std::conditional<std::is_same<C2, float2>::value, float2, double2>::type;
+  __shared__ LT y1[NP * N * HH];
+  __shared__ LT y2[NP * NP * HH];
   __shared__ int qof[NP * NP * NP];  // block index -> surface sample, -1 inside
   const int slot = blockIdx.x;
   const int node = v.trg_node[slot];
@@ -436,7 +519,7 @@ __global__ __launch_bounds__(64) void k_fft_inv(VlistArgs v, const C2* espec) {
           double2 acc = make_double2(0, 0);
 #pragma unroll
           for (int ka = 0; ka < N; ka++) acc = cadd(acc, cmul(x[ka], w[((a + O) * ka) % N]));
-          y1[a * N * HH + e] = acc;
+          store_c(&y1[a * N * HH + e], acc);
         }
       }
     } else {
@@ -454,12 +537,19 @@ __global__ __launch_bounds__(64) void k_fft_inv(VlistArgs v, const C2* espec) {
         double2 x[N];
 #pragma unroll
         for (int ka = 0; ka < N; ka++) x[ka] = load_c(half ? &r1[ka] : &r0[ka]);
+        if constexpr (kFast) {
+          double2 o6[NP];
+          pad12_inv(x, w, o6);
 #pragma unroll
-        for (int a = 0; a < NP; a++) {
-          double2 acc = make_double2(0, 0);
+          for (int a = 0; a < NP; a++) store_c(&y1[a * N * HH + e], o6[a]);
+        } else {
 #pragma unroll
-          for (int ka = 0; ka < N; ka++) acc = cadd(acc, cmul(x[ka], w[((a + O) * ka) % N]));
-          y1[a * N * HH + e] = acc;
+          for (int a = 0; a < NP; a++) {
+            double2 acc = make_double2(0, 0);
+#pragma unroll
+            for (int ka = 0; ka < N; ka++) acc = cadd(acc, cmul(x[ka], w[((a + O) * ka) % N]));
+            store_c(&y1[a * N * HH + e], acc);
+          }
         }
       }
     }
@@ -469,13 +559,20 @@ __global__ __launch_bounds__(64) void k_fft_inv(VlistArgs v, const C2* espec) {
       const int a = e / HH, kc = e % HH;
       double2 x[N];
 #pragma unroll
-      for (int kb = 0; kb < N; kb++) x[kb] = y1[(a * N + kb) * HH + kc];
+      for (int kb = 0; kb < N; kb++) x[kb] = load_c(&y1[(a * N + kb) * HH + kc]);
+      if constexpr (kFast) {
+        double2 o6[NP];
+        pad12_inv(x, w, o6);
 #pragma unroll
-      for (int b = 0; b < NP; b++) {
-        double2 acc = make_double2(0, 0);
+        for (int b = 0; b < NP; b++) store_c(&y2[(a * NP + b) * HH + kc], o6[b]);
+      } else {
 #pragma unroll
-        for (int kb = 0; kb < N; kb++) acc = cadd(acc, cmul(x[kb], w[((b + O) * kb) % N]));
-        y2[(a * NP + b) * HH + kc] = acc;
+        for (int b = 0; b < NP; b++) {
+          double2 acc = make_double2(0, 0);
+#pragma unroll
+          for (int kb = 0; kb < N; kb++) acc = cadd(acc, cmul(x[kb], w[((b + O) * kb) % N]));
+          store_c(&y2[(a * NP + b) * HH + kc], acc);
+        }
       }
     }
     __syncthreads();
@@ -483,7 +580,7 @@ __global__ __launch_bounds__(64) void k_fft_inv(VlistArgs v, const C2* espec) {
     for (int ab = lane; ab < NP * NP; ab += 64) {
       double2 yy[HH];
 #pragma unroll
-      for (int kc = 0; kc < HH; kc++) yy[kc] = y2[ab * HH + kc];
+      for (int kc = 0; kc < HH; kc++) yy[kc] = load_c(&y2[ab * HH + kc]);
 #pragma unroll
       for (int c = 0; c < NP; c++) {
         const int q = qof[ab * NP + c];
