@@ -46,6 +46,9 @@ struct SurfArgs {
   const double4* geom;
   const double* sam; int n_sam;
   double* out; int sz_out;
+  // tasks of one leaf each, as (leaf, first source, source count): saves the
+  // task -> item -> range lookups (mixed S2M; null elsewhere)
+  const int4* leaf;
 };
 
 struct GemmArgs {

@@ -158,12 +158,13 @@ int FMM3d::build_tasks() {
   }
 
   // --- S2M (:557-577) and X-list S2L (:1241-1257) ---
-  s2m_tasks_h_.clear(); s2l_tasks_h_.clear(); surf_items_h_.clear();
+  s2m_tasks_h_.clear(); s2l_tasks_h_.clear(); surf_items_h_.clear(); s2m_leaf_h_.clear();
   for (int B = 1; B < N; B++)
     if ((t.tag[B] & TAG_CBTR) && t.leaf(B)) {
       int ib = int(surf_items_h_.size());
       surf_items_h_.push_back(B);
       s2m_tasks_h_.push_back(SurfTask{B, ib, ib + 1});
+      s2m_leaf_h_.push_back(make_int4(B, t.src_beg[B], t.src_num[B], 0));
       work_.s2m_pairs += double(t.src_num[B]) * op.n_sam[UC];
     }
   for (int B = 0; B < N; B++) {
@@ -403,6 +404,7 @@ int FMM3d::upload_state() {
   KIFMM_UP(near_chunks_d_, near_chunks_h_, "upload near chunks");
   KIFMM_UP(near_items_d_, near_items_h_, "upload near items");
   KIFMM_UP(s2m_tasks_d_, s2m_tasks_h_, "upload S2M tasks");
+  KIFMM_UP(s2m_leaf_d_, s2m_leaf_h_, "upload S2M tasks");
   KIFMM_UP(s2l_tasks_d_, s2l_tasks_h_, "upload S2L tasks");
   KIFMM_UP(surf_items_d_, surf_items_h_, "upload surface items");
   KIFMM_UP(lvl_idx_d_, lvl_idx_h_, "upload level columns");
@@ -648,6 +650,7 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
   // up pass (:480-619)
   mark(PH_UPW);
   sa.tasks = s2m_tasks_d_.as<SurfTask>(); sa.ntasks = int(s2m_tasks_h_.size());
+  sa.leaf = mixed ? s2m_leaf_d_.as<int4>() : nullptr;
   sa.sam = sam_d_[UC].as<double>(); sa.n_sam = op.n_sam[UC];
   sa.out = uc_d_.as<double>(); sa.sz_out = op.dat(UC);
   if ((rc = check(mixed ? launch_surface_mixed(kt_, sa, pu, s) : launch_surface(kt_, sa, pu, s), "S2M"))) return rc;
@@ -664,6 +667,7 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
   // X list S2L (:1233-1265)
   mark(PH_XLIST);
   sa.tasks = s2l_tasks_d_.as<SurfTask>(); sa.ntasks = int(s2l_tasks_h_.size());
+  sa.leaf = nullptr;
   sa.sam = sam_d_[DC].as<double>(); sa.n_sam = op.n_sam[DC];
   sa.out = dc_d_.as<double>(); sa.sz_out = op.dat(DC);
   if ((rc = check(mixed ? launch_surface_mixed(kt_, sa, pu, s) : launch_surface(kt_, sa, pu, s), "S2L"))) return rc;

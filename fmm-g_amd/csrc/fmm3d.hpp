@@ -163,6 +163,7 @@ class FMM3d {
   std::vector<NearChunk> near_chunks_h_;
   std::vector<int> near_items_h_;
   std::vector<SurfTask> s2m_tasks_h_, s2l_tasks_h_;
+  std::vector<int4> s2m_leaf_h_;  // S2M tasks as (leaf, first source, source count)
   std::vector<int> surf_items_h_;
   VGroups vfwd_, vadj_;
   // adjoint tables (built on the first evaluateAdjoint)
@@ -177,7 +178,7 @@ class FMM3d {
   DeviceBuffer spos_d_, snor_d_, tpos_d_, sperm_d_, tperm_d_, sden_d_, tval_d_;
   DeviceBuffer geom_d_, depth_d_, sbeg_d_, snum_d_;
   DeviceBuffer ue_d_, uc_d_, dc_d_, de_d_;
-  DeviceBuffer near_chunks_d_, near_items_d_, s2m_tasks_d_, s2l_tasks_d_, surf_items_d_;
+  DeviceBuffer near_chunks_d_, near_items_d_, s2m_tasks_d_, s2l_tasks_d_, surf_items_d_, s2m_leaf_d_;
   DeviceBuffer lvl_idx_d_;
   DeviceBuffer sam_d_[4], shell_d_, twiddle_d_, deg_d_, ones_d_, scales_d_;
   DeviceBuffer pup_d_, pdn_d_, m2m_d_, l2l_d_, m2l_d_;

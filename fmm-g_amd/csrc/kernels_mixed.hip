@@ -312,8 +312,15 @@ __global__ __launch_bounds__(kSurfMx) void k_surface_mx(SurfArgs a, KParams<floa
   __shared__ float sp[kSurfMx * 3];
   __shared__ float sn[NOR ? kSurfMx * 3 : 1];
   __shared__ float sv[kSurfMx * SD];
-  const SurfTask tk = a.tasks[blockIdx.x];
-  const double4 g = a.geom[tk.node];
+  int node, ib, ie, sb1 = 0, sn1 = 0;
+  if (a.leaf) {
+    const int4 L = a.leaf[blockIdx.x];
+    node = L.x; sb1 = L.y; sn1 = L.z; ib = 0; ie = 1;
+  } else {
+    const SurfTask tk = a.tasks[blockIdx.x];
+    node = tk.node; ib = tk.ibeg; ie = tk.iend;
+  }
+  const double4 g = a.geom[node];
   const int lane = threadIdx.x;
   float tp[2][3];
   double acc[2][TD];
@@ -325,9 +332,12 @@ __global__ __launch_bounds__(kSurfMx) void k_surface_mx(SurfArgs a, KParams<floa
     for (int d = 0; d < 3; d++) tp[u][d] = float(g.w * a.sam[3 * qq + d]);
     for (int t = 0; t < TD; t++) acc[u][t] = 0.0;
   }
-  for (int it = tk.ibeg; it < tk.iend; it++) {
-    const int node = a.items[it];
-    const int b = a.src_beg[node], ns = a.src_num[node];
+  for (int it = ib; it < ie; it++) {
+    int b = sb1, ns = sn1;
+    if (!a.leaf) {
+      const int src = a.items[it];
+      b = a.src_beg[src]; ns = a.src_num[src];
+    }
     for (int base = 0; base < ns; base += kSurfMx) {
       const int m = min(kSurfMx, ns - base);
       __syncthreads();
@@ -355,7 +365,7 @@ __global__ __launch_bounds__(kSurfMx) void k_surface_mx(SurfArgs a, KParams<floa
       }
     }
   }
-  double* o = a.out + int64_t(tk.node) * a.sz_out;
+  double* o = a.out + int64_t(node) * a.sz_out;
   for (int u = 0; u < 2; u++) {
     if (!act[u]) continue;
     const int q = lane + u * kSurfMx;
