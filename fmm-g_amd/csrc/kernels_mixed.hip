@@ -594,11 +594,13 @@ hipError_t launch_hadamard_mixed(const VlistArgs& v, const float2* dspec, const 
 
 // ---------------------------------------------------------------------------
 // Mixed path: FP64 matrix cores (v_mfma_f64_16x16x4f64, FMA, no order
-// contract).  160 x 64 tile per workgroup (M = 152 or 296 needs 1 or 2 row
-// tiles, 5% or 8% padding); wave w owns columns [16w, 16w+16) and all ten
-// 16-row blocks; K staged through LDS in slabs of 16 with the next slab
-// prefetched into registers while the current one feeds 40 MFMAs per wave
-// (168 VGPRs: 3 waves per SIMD).
+// contract).  160 x 64 tile per workgroup of 8 waves (M = 152 or 296 needs 1 or
+// 2 row tiles, 5% or 8% padding); wave w owns columns [16 (w % 4), +16) and the
+// five 16-row blocks of row half w / 4.  K is staged through LDS in slabs of 16
+// with the next slab prefetched into registers while the current one feeds 20
+// MFMAs per wave.  108 VGPRs and 33 KB of LDS: 4 waves per SIMD (the same tile
+// on 4 waves of ten row blocks each needed 168 VGPRs, 3 waves per SIMD, and was
+// 13% slower on the down pass).
 // With ksplit > 1, blockIdx.z = matrix * ksplit + K chunk and the partial
 // products are added to C with FP64 atomics, so a level with few columns still
 // spreads over enough workgroups (C must then hold its starting value: zero
@@ -607,18 +609,16 @@ hipError_t launch_hadamard_mixed(const VlistArgs& v, const float2* dspec, const 
 // B[l/16][l%16], and its result register r holds D[4r + l/16][l%16] (rows
 // interleaved across the four lane groups).
 typedef double kd4 __attribute__((ext_vector_type(4)));
-constexpr int GM_M = 160, GM_N = 64, GM_K = 16, GM_MB = GM_M / 16;
-constexpr int GM_RA = GM_M * GM_K / 256, GM_RB = GM_K * GM_N / 256;
+constexpr int GM_M = 160, GM_N = 64, GM_K = 16, GM_T = 512, GM_MB = GM_M / 32;
+constexpr int GM_RA = GM_M * GM_K / GM_T, GM_RB = GM_K * GM_N / GM_T;
 
-__global__ __launch_bounds__(256, 3) void k_gemm_mfma(GemmArgs g) {
-  // row pitches = 16 mod 32 doubles: the four row groups of an operand read
-  // fall on disjoint bank halves
+__global__ __launch_bounds__(GM_T, 2) void k_gemm_mfma(GemmArgs g) {
   __shared__ double As[GM_K][GM_M + 16];
   __shared__ double Bs[GM_K][GM_N + 16];
   __shared__ int64_t bcol[GM_N];
   const bool split = g.ksplit > 1;
-  const int zk = split ? int(blockIdx.z) % g.ksplit : 0;  // K chunk
-  const int z = split ? int(blockIdx.z) / g.ksplit : int(blockIdx.z);  // matrix
+  const int zk = split ? int(blockIdx.z) % g.ksplit : 0;
+  const int z = split ? int(blockIdx.z) / g.ksplit : int(blockIdx.z);
   const int cbeg = g.zoff ? g.zoff[z] : 0, cend = g.zoff ? g.zoff[z + 1] : g.ncol;
   const int n0 = cbeg + int(blockIdx.y) * GM_N;
   if (n0 >= cend) return;
@@ -628,6 +628,7 @@ __global__ __launch_bounds__(256, 3) void k_gemm_mfma(GemmArgs g) {
   if (kbeg >= kend) return;
   const int m0 = blockIdx.x * GM_M;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
+  const int wc = wave & 3, wr = wave >> 2;  // column block, row half
   if (tid < GM_N) {
     const int j = n0 + tid;
     bcol[tid] = j < cend ? int64_t(g.bidx[j]) * g.ldb : int64_t(-1);
@@ -641,13 +642,12 @@ __global__ __launch_bounds__(256, 3) void k_gemm_mfma(GemmArgs g) {
   auto fetch = [&](int k0) {
 #pragma unroll
     for (int u = 0; u < GM_RA; u++) {
-      const int e = tid + 256 * u;
+      const int e = tid + GM_T * u;
       const int m = m0 + e % GM_M, k = k0 + e / GM_M;
       ra[u] = (m < g.M && k < kend) ? A[int64_t(k) * g.M + m] : 0.0;
     }
-    // B: lane owns column tid/4 and the GM_RB consecutive k from GM_RB*(tid%4)
-    const int64_t col = bcol[tid >> 2];
-    const int kq0 = k0 + GM_RB * (tid & 3);
+    const int64_t col = bcol[tid >> 3];
+    const int kq0 = k0 + GM_RB * (tid & 7);
 #pragma unroll
     for (int u = 0; u < GM_RB; u++) {
       const int kq = kq0 + u;
@@ -658,30 +658,30 @@ __global__ __launch_bounds__(256, 3) void k_gemm_mfma(GemmArgs g) {
   for (int k0 = kbeg; k0 < kend; k0 += GM_K) {
 #pragma unroll
     for (int u = 0; u < GM_RA; u++) {
-      const int e = tid + 256 * u;
+      const int e = tid + GM_T * u;
       As[e / GM_M][e % GM_M] = ra[u];
     }
 #pragma unroll
-    for (int u = 0; u < GM_RB; u++) Bs[GM_RB * (tid & 3) + u][tid >> 2] = rb[u];
+    for (int u = 0; u < GM_RB; u++) Bs[GM_RB * (tid & 7) + u][tid >> 3] = rb[u];
     __syncthreads();
     if (k0 + GM_K < kend) fetch(k0 + GM_K);
 #pragma unroll
     for (int ks = 0; ks < GM_K / 4; ks++) {
-      const double b = Bs[4 * ks + lk][16 * wave + li];
+      const double b = Bs[4 * ks + lk][16 * wc + li];
 #pragma unroll
       for (int mi = 0; mi < GM_MB; mi++)
-        acc[mi] = __builtin_amdgcn_mfma_f64_16x16x4f64(As[4 * ks + lk][16 * mi + li], b, acc[mi], 0, 0, 0);
+        acc[mi] = __builtin_amdgcn_mfma_f64_16x16x4f64(As[4 * ks + lk][80 * wr + 16 * mi + li], b, acc[mi], 0, 0, 0);
     }
     __syncthreads();
   }
-  const int n = n0 + 16 * wave + li;
+  const int n = n0 + 16 * wc + li;
   if (n >= cend) return;
   double* col = g.C + int64_t(g.cidx[n]) * g.ldc;
 #pragma unroll
   for (int mi = 0; mi < GM_MB; mi++)
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      const int m = m0 + 16 * mi + 4 * r + lk;
+      const int m = m0 + 80 * wr + 16 * mi + 4 * r + lk;
       if (m >= g.M) continue;
       const double v = g.rscale[m % g.rsd] * acc[mi][r];
       if (split) unsafeAtomicAdd(col + m, v);
@@ -695,7 +695,7 @@ static hipError_t launch_gemm_mfma(const GemmArgs& g, hipStream_t s) {
   if (split && (g.kchunk <= 0 || int64_t(g.ksplit) * g.kchunk < g.K)) return hipErrorInvalidValue;
   const int nz = (g.zoff ? g.nz : 1) * (split ? g.ksplit : 1);
   dim3 grid(unsigned((g.M + GM_M - 1) / GM_M), unsigned((g.max_zcol + GM_N - 1) / GM_N), unsigned(nz));
-  k_gemm_mfma<<<grid, dim3(256), 0, s>>>(g);
+  k_gemm_mfma<<<grid, dim3(GM_T), 0, s>>>(g);
   return hipGetLastError();
 }
 
