@@ -11,8 +11,13 @@
 namespace kifmm {
 
 // A chunk of at most kNearChunk targets of one leaf and its source items.
+// own: position (from ibeg) of the leaf's own exact item, -1 if absent.
+// unchk: 1 if no other item has a source within 1e-7 of the chunk's targets
+// (bounding boxes), so that only the own item's sources can fall under the
+// 1e-8 pair cut; 0 (the default) keeps the cut on every pair.
 struct NearChunk {
   int node, tbeg, tnum, ibeg, iend;
+  int own = -1, unchk = 0;
 };
 constexpr int kNearChunk = 128;
 constexpr int kPairStride = 224;  // 216 halo slots rounded up to a multiple of 8

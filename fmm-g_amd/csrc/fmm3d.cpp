@@ -120,6 +120,21 @@ int FMM3d::build_tasks() {
 
   // --- near field: U, W (S2T or M2T), X (S2T fallback), L2T ---
   near_chunks_h_.clear(); near_items_h_.clear();
+  // bounding boxes of every leaf's sources (the near kernel of the mixed path
+  // applies the 1e-8 pair cut only where a source can come that close)
+  std::vector<double> sbox(size_t(N) * 6);
+  for (int A = 0; A < N; A++) {
+    double* bx = &sbox[size_t(A) * 6];
+    for (int d = 0; d < 3; d++) { bx[d] = 1e300; bx[3 + d] = -1e300; }
+    if (t.src_beg[A] < 0) continue;
+    for (int k = 0; k < t.src_num[A]; k++) {
+      const int64_t o = t.src_perm[size_t(t.src_beg[A]) + k];
+      for (int d = 0; d < 3; d++) {
+        bx[d] = std::min(bx[d], spos_[3 * o + d]);
+        bx[3 + d] = std::max(bx[3 + d], spos_[3 * o + d]);
+      }
+    }
+  }
   for (int B = 0; B < N; B++) {
     if (!(t.tag[B] & TAG_EVTR) || !t.leaf(B)) continue;
     const int ibeg = int(near_items_h_.size());
@@ -151,8 +166,32 @@ int FMM3d::build_tasks() {
       surf += op.n_sam[DE];
     }
     const int iend = int(near_items_h_.size());
-    for (int c = 0; c < tn; c += kNearChunk)
-      near_chunks_h_.push_back(NearChunk{B, t.trg_beg[B] + c, std::min(kNearChunk, tn - c), ibeg, iend});
+    for (int c = 0; c < tn; c += kNearChunk) {
+      NearChunk ch{B, t.trg_beg[B] + c, std::min(kNearChunk, tn - c), ibeg, iend};
+      double tb[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
+      for (int k = 0; k < ch.tnum; k++) {
+        const int64_t o = t.trg_perm[size_t(ch.tbeg) + k];
+        for (int d = 0; d < 3; d++) {
+          tb[d] = std::min(tb[d], tpos_[3 * o + d]);
+          tb[3 + d] = std::max(tb[3 + d], tpos_[3 * o + d]);
+        }
+      }
+      bool sep = true;
+      for (int i = ibeg; i < iend && sep; i++) {
+        const int code = near_items_h_[i], A = code >> 2;
+        if ((code & 3) != ITEM_EXACT) continue;  // surfaces of non-adjacent nodes or the DE surface
+        if (A == B && ch.own < 0) { ch.own = i - ibeg; continue; }
+        const double* bx = &sbox[size_t(A) * 6];
+        double d2 = 0;
+        for (int d = 0; d < 3; d++) {
+          const double g = std::max(0.0, std::max(bx[d] - tb[3 + d], tb[d] - bx[3 + d]));
+          d2 += g * g;
+        }
+        if (d2 < 1e-14) sep = false;  // within 1e-7: keep the cut on every pair
+      }
+      ch.unchk = sep ? 1 : 0;
+      near_chunks_h_.push_back(ch);
+    }
     work_.near_exact_pairs += double(tn) * exact;
     work_.near_pairs += double(tn) * (exact + surf);
   }

@@ -177,10 +177,13 @@ __global__ __launch_bounds__(kNearChunk) void k_near_mx(NearArgs a, KParams<floa
 // of the block (source count rounded up to 4, split S ways in multiples of 4),
 // unrolled by 4.  Padding entries have zero weight and lie 1e18 away, so they
 // add exact zeros.
-// Pairs closer than the reference's 1e-8 cut contribute nothing.  (Skipping
-// that test where the host can prove no such pair exists was measured slower.)
-// The FP32 partial sums per item are added in FP64, near (exact, UE) and far
-// (DE) parts separately.
+// Pairs closer than the reference's 1e-8 cut contribute nothing.  Only a
+// source of the target leaf itself can come that close, unless the host found
+// another item's sources within 1e-7 of the chunk's targets (then unchk = 0):
+// blocks without the own item's sources use the sweep without the test (and
+// the bare v_rsq_f32: r^2 >= 1e-16 there, a normal float).  The FP32 partial
+// sums per block are added in FP64, near (exact, UE) and far (DE) parts
+// separately.
 __device__ __forceinline__ void lap_sweep(const float4* s4, int j0, int j1, float tx, float ty, float tz,
                                           float& acc0, float& acc1) {
   const float eps2 = float(kMinDif * kMinDif);
@@ -194,6 +197,22 @@ __device__ __forceinline__ void lap_sweep(const float4* s4, int j0, int j1, floa
       const float r1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
       acc0 = fmaf(r0 < eps2 ? 0.f : rsqrtf(r0), s0.w, acc0);
       acc1 = fmaf(r1 < eps2 ? 0.f : rsqrtf(r1), s1.w, acc1);
+    }
+  }
+}
+
+__device__ __forceinline__ void lap_sweep_nc(const float4* s4, int j0, int j1, float tx, float ty, float tz,
+                                             float& acc0, float& acc1) {
+  for (int j = j0; j < j1; j += 4) {
+#pragma unroll
+    for (int u = 0; u < 4; u += 2) {
+      const float4 s0 = s4[j + u], s1 = s4[j + u + 1];
+      const float dx0 = tx - s0.x, dy0 = ty - s0.y, dz0 = tz - s0.z;
+      const float dx1 = tx - s1.x, dy1 = ty - s1.y, dz1 = tz - s1.z;
+      const float r0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
+      const float r1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
+      acc0 = fmaf(__builtin_amdgcn_rsqf(r0), s0.w, acc0);
+      acc1 = fmaf(__builtin_amdgcn_rsqf(r1), s1.w, acc1);
     }
   }
 }
@@ -247,8 +266,12 @@ __global__ __launch_bounds__(kNearChunk) void k_near_lap_mx(NearArgs a, float pa
       if (lane == 0) pre[0] = 0;
       __syncthreads();
       const int total = pre[nw];
+      // slots of the target leaf's own item in this window (empty if absent)
+      const int ow = ch.own >= 0 ? ch.ibeg + ch.own - w0 : -1;
+      const int own0 = ow >= 0 && ow < nw ? pre[ow] : 0, own1 = ow >= 0 && ow < nw ? pre[ow + 1] : 0;
       for (int base = 0; base < total; base += kNearChunk) {
         const int m = min(kNearChunk, total - base);
+        const bool cut = !ch.unchk || (own0 < base + m && own1 > base);
         __syncthreads();
         float4 v = make_float4(1e18f, 0.f, 0.f, 0.f);
         if (lane < m) {
@@ -280,7 +303,8 @@ __global__ __launch_bounds__(kNearChunk) void k_near_lap_mx(NearArgs a, float pa
           const int L = ((m4 + S - 1) / S + 3) & ~3;
           const int j0 = sub * L, j1 = min(j0 + L, m4);
           float acc0 = 0.f, acc1 = 0.f;
-          lap_sweep(s4, j0, j1, tx, ty, tz, acc0, acc1);
+          if (cut) lap_sweep(s4, j0, j1, tx, ty, tz, acc0, acc1);
+          else lap_sweep_nc(s4, j0, j1, tx, ty, tz, acc0, acc1);
           part[pass] += double(acc0 + acc1);
         }
       }

@@ -181,6 +181,32 @@ def test_mixed_precision(kt, np_, coefs, gen, tol):
     assert out["mixed_err"] <= out["fp64_err"] + 2 * d
 
 
+def test_mixed_near_cut_across_leaves():
+    """The mixed near field drops the 1e-8 pair test where the host proves no
+    pair can need it (only the target leaf's own sources can).  Points 5e-9
+    apart on either side of the plane x = 1/2 (a leaf face at every level)
+    must still have their pair excluded, as the FP64 path (test on every pair)
+    does; a missed cut would add terms of order 1e7."""
+    rng = np.random.default_rng(95)
+    base = W.uniform(6000, 96)
+    yz = rng.random((6, 2)) * 0.8 + 0.1
+    left = np.column_stack([np.full(6, 0.5 - 2.5e-9), yz])
+    right = np.column_stack([np.full(6, 0.5 + 2.5e-9), yz])
+    pos = np.concatenate([base, left, right, base[:20]])  # plus duplicates in one leaf
+    den = W.densities(len(pos), 1, 97)
+    print("case mixed near cut n=%d" % len(pos), flush=True)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        f = kifmm.FMM3d(111, [1.0, 0.25], 4, 32, 30, precision=prec)
+        f.set_points(pos)
+        f.setup()
+        out[prec] = f.evaluate(den)
+    assert np.all(np.isfinite(out["mixed"]))
+    assert rel(out["mixed"], out["fp64"]) < 3e-7
+    k = slice(6000, 6012)
+    assert rel(out["mixed"][k], out["fp64"][k]) < 1e-5
+
+
 def test_batched_problems_multi_rhs():
     """The many-problem workload, reduced: independent Navier problems (np 8,
     ptsMax 120, maxLevel 10, nu 0.3), each with 3 right-hand sides through
