@@ -371,7 +371,8 @@ __global__ __launch_bounds__(kSurfMx) void k_surface_mx(SurfArgs a, KParams<floa
         sp[3 * lane + 1] = float(a.spos[3 * gg + 1] - g.y);
         sp[3 * lane + 2] = float(a.spos[3 * gg + 2] - g.z);
         if constexpr (NOR) for (int d = 0; d < 3; d++) sn[3 * lane + d] = float(a.snor[3 * gg + d]);
-        for (int s = 0; s < SD; s++) sv[lane * SD + s] = float(a.sden[gg * SD + s]);
+        // Laplace: the 1/(4 pi) factor goes with the density
+        for (int s = 0; s < SD; s++) sv[lane * SD + s] = float(a.sden[gg * SD + s]) * (KT == 111 ? pu.a : 1.f);
       }
       __syncthreads();
 #pragma unroll
@@ -379,11 +380,22 @@ __global__ __launch_bounds__(kSurfMx) void k_surface_mx(SurfArgs a, KParams<floa
         if (!act[u]) continue;
         float part[TD];
         for (int t = 0; t < TD; t++) part[t] = 0.0f;
-        for (int j = 0; j < m; j++) {
-          float nx = 0, ny = 0, nz = 0;
-          if constexpr (NOR) { nx = sn[3 * j]; ny = sn[3 * j + 1]; nz = sn[3 * j + 2]; }
-          pair_f32<KT>(pu, tp[u][0] - sp[3 * j], tp[u][1] - sp[3 * j + 1], tp[u][2] - sp[3 * j + 2], nx, ny, nz,
-                       &sv[j * SD], part);
+        if constexpr (KT == 111) {
+          // check surfaces never meet their sources: the upward check surface
+          // lies outside the leaf, and an X-list leaf is not adjacent to the
+          // target node whose downward check surface is its boundary, so no
+          // pair is within the 1e-8 cut and r^2 is a normal float
+          for (int j = 0; j < m; j++) {
+            const float dx = tp[u][0] - sp[3 * j], dy = tp[u][1] - sp[3 * j + 1], dz = tp[u][2] - sp[3 * j + 2];
+            part[0] = fmaf(__builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz), sv[j], part[0]);
+          }
+        } else {
+          for (int j = 0; j < m; j++) {
+            float nx = 0, ny = 0, nz = 0;
+            if constexpr (NOR) { nx = sn[3 * j]; ny = sn[3 * j + 1]; nz = sn[3 * j + 2]; }
+            pair_f32<KT>(pu, tp[u][0] - sp[3 * j], tp[u][1] - sp[3 * j + 1], tp[u][2] - sp[3 * j + 2], nx, ny, nz,
+                         &sv[j * SD], part);
+          }
         }
         for (int t = 0; t < TD; t++) acc[u][t] += double(part[t]);
       }
