@@ -65,8 +65,11 @@ struct GemmArgs {
   // columns [zoff[z], zoff[z+1]); zoff == nullptr means one matrix, all columns
   const int* zoff; int nz; int64_t a_zstride; int max_zcol;
   // split-K (mixed path only): blockIdx.z = matrix * ksplit + c sums k in
-  // [c*kchunk, (c+1)*kchunk) and adds its partial product to C atomically
+  // [c*kchunk, (c+1)*kchunk).  With work (ksplit * ncol * M doubles) the partial
+  // products go there and a second kernel sums them in chunk order into C
+  // (deterministic); without it they are added to C atomically.
   int ksplit; int kchunk;
+  double* work;
 };
 
 struct VlistArgs {

@@ -485,6 +485,26 @@ int FMM3d::upload_state() {
     wup_d_.release();
   }
   KIFMM_UP(l2l_d_, op.l2l, "upload L2L");
+  if (precision_ == PREC_MIXED) {
+    // split-K workspace: the same split rule as the evaluate over every step
+    size_t need = 1;
+    for (const auto* steps : {&up_steps_, &dn_steps_})
+      for (const GemmStep& st : *steps) {
+        GemmArgs g{};
+        switch (st.mat) {
+          case 0: g.M = op.dat(UC); g.K = 8 * op.dat(UE); break;
+          case 1: g.M = op.dat(UE); g.K = op.dat(UC); break;
+          case 8: g.M = op.dat(UE); g.K = 8 * op.dat(UE); break;
+          case 2: g.M = op.dat(DE); g.K = op.dat(DC); break;
+          default: g.M = op.dat(DC); g.K = op.dat(DE);
+        }
+        g.ncol = st.ncol; g.max_zcol = st.ncol;
+        if (st.zoff_off >= 0) { g.zoff = &st.ncol; g.nz = 8; g.max_zcol = st.max_zcol; }  // only tested for null
+        split_k(g);
+        if (g.ksplit > 1) need = std::max(need, size_t(g.ksplit) * size_t(g.ncol) * size_t(g.M));
+      }
+    if ((rc = reserve_splitk(need))) return rc;
+  }
   KIFMM_UP(m2l_d_, op.m2l, "upload M2L");
   // V list
   if ((rc = upload_vgroups(vfwd_))) return rc;
@@ -563,6 +583,18 @@ int FMM3d::run_vlist(VlistArgs& v, hipStream_t s, bool timed) {
 // ---------------------------------------------------------------------------
 // evaluate: fmm3d_eval_mpi.cpp:396-1515
 // ---------------------------------------------------------------------------
+int FMM3d::reserve_splitk(size_t doubles) {
+  if (doubles * sizeof(double) <= splitk_d_.bytes()) return 0;
+  return check(splitk_d_.alloc(doubles * sizeof(double)), "alloc split-K workspace");
+}
+
+// The workspace for a split launch, or null (atomic adds) if it does not fit.
+double* FMM3d::splitk_for(const GemmArgs& g) {
+  if (g.ksplit <= 1) return nullptr;
+  const size_t need = size_t(g.ksplit) * size_t(g.ncol) * size_t(g.M) * sizeof(double);
+  return need <= splitk_d_.bytes() ? splitk_d_.as<double>() : nullptr;
+}
+
 void split_k(GemmArgs& g) {
   // A launch's time is its tile count over the CUs times the serial K loop
   // (~3 us per 16-deep slab).  Splitting K pays where that loop is long and the
@@ -681,7 +713,7 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
       // mixed path: launches with very few workgroups split K into chunks of
       // whole 16-deep slabs; the partial products are added atomically into
       // outputs that start at zero (pinv, M2M) or accumulate (L2L)
-      if (mixed) split_k(g);
+      if (mixed) { split_k(g); g.work = splitk_for(g); }
       int r = check(mixed ? launch_gemm_fma(g, s) : launch_gemm(g, s), "level GEMM");
       if (r) return r;
     }

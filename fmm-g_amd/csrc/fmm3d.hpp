@@ -158,6 +158,9 @@ class FMM3d {
     int max_zcol = 0;
   };
   std::vector<GemmStep> up_steps_, dn_steps_;
+  DeviceBuffer splitk_d_;  // mixed path: split-K partial products (ordered sum, no atomics)
+  int reserve_splitk(size_t doubles);
+  double* splitk_for(const GemmArgs& g);
   std::vector<int> lvl_idx_h_;
   std::vector<double> scales_h_;
   std::vector<NearChunk> near_chunks_h_;

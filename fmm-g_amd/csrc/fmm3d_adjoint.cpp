@@ -196,6 +196,24 @@ int FMM3d::build_adjoint() {
         return rc;
     }
   }
+  if (precision_ == PREC_MIXED) {
+    // split-K workspace for the transposed steps (the same rule as the pass)
+    size_t need = 1;
+    for (const auto* steps : {&dn_t_steps_, &up_t_steps_})
+      for (const GemmStep& st : *steps) {
+        GemmArgs g{};
+        switch (st.mat) {
+          case 4: g.M = 8 * op.dat(UE); g.K = op.dat(UC); break;
+          case 5: g.M = op.dat(UC); g.K = op.dat(UE); break;
+          case 6: g.M = op.dat(DC); g.K = op.dat(DE); break;
+          default: g.M = op.dat(DE); g.K = 8 * op.dat(DC);
+        }
+        g.ncol = st.ncol; g.max_zcol = st.ncol;
+        split_k(g);
+        if (g.ksplit > 1) need = std::max(need, size_t(g.ksplit) * size_t(g.ncol) * size_t(g.M));
+      }
+    if ((rc = reserve_splitk(need))) return rc;
+  }
   adj_ready_ = true;
   return 0;
 }
@@ -258,7 +276,7 @@ int FMM3d::evaluateAdjoint(const double* d_trg_in, double* d_src_out, hipStream_
       g.ncol = stp.ncol; g.accumulate = 1;
       // every adjoint step accumulates, so K chunks may add atomically; the FP64
       // mode keeps one workgroup per output tile and is deterministic
-      if (precision_ == PREC_MIXED) split_k(g);
+      if (precision_ == PREC_MIXED) { split_k(g); g.work = splitk_for(g); }
       int r = check(launch_gemm_fma(g, s), "adjoint level GEMM");
       if (r) return r;
     }

@@ -720,9 +720,22 @@ __global__ __launch_bounds__(GM_T, 2) void k_gemm_mfma(GemmArgs g) {
       const int m = m0 + 80 * wr + 16 * mi + 4 * r + lk;
       if (m >= g.M) continue;
       const double v = g.rscale[m % g.rsd] * acc[mi][r];
-      if (split) unsafeAtomicAdd(col + m, v);
+      if (split && g.work) g.work[(int64_t(zk) * g.ncol + n) * g.M + m] = v;
+      else if (split) unsafeAtomicAdd(col + m, v);
       else col[m] = g.accumulate ? col[m] + v : v;
     }
+}
+
+// Sum of the split-K partial products in chunk order (one thread per output).
+__global__ void k_splitk_reduce(const double* __restrict__ work, int ks, int ncol, int M, const int* __restrict__ cidx,
+                                double* __restrict__ C, int64_t ldc, int accumulate) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= int64_t(ncol) * M) return;
+  const int n = int(i / M), m = int(i % M);
+  double v = 0.0;
+  for (int z = 0; z < ks; z++) v += work[(int64_t(z) * ncol + n) * M + m];
+  double* dst = C + int64_t(cidx[n]) * ldc + m;
+  *dst = accumulate ? *dst + v : v;
 }
 
 static hipError_t launch_gemm_mfma(const GemmArgs& g, hipStream_t s) {
@@ -732,6 +745,11 @@ static hipError_t launch_gemm_mfma(const GemmArgs& g, hipStream_t s) {
   const int nz = (g.zoff ? g.nz : 1) * (split ? g.ksplit : 1);
   dim3 grid(unsigned((g.M + GM_M - 1) / GM_M), unsigned((g.max_zcol + GM_N - 1) / GM_N), unsigned(nz));
   k_gemm_mfma<<<grid, dim3(GM_T), 0, s>>>(g);
+  if (split && g.work) {
+    const int64_t n = int64_t(g.ncol) * g.M;
+    k_splitk_reduce<<<dim3(unsigned((n + 255) / 256)), dim3(256), 0, s>>>(g.work, g.ksplit, g.ncol, g.M, g.cidx, g.C,
+                                                                           g.ldc, g.accumulate);
+  }
   return hipGetLastError();
 }
 

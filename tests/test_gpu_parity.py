@@ -232,13 +232,34 @@ def test_batched_problems_multi_rhs():
             assert rel(vals[r], ref) <= PARITY
 
 
-@pytest.mark.parametrize("precision,bar", [("fp64", 0.0), ("mixed", 1e-6)])
+def test_mixed_deterministic():
+    """The mixed path is bitwise reproducible: the same densities give the same
+    potentials on one handle, on a second handle, and through the transposed
+    pass (split-K GEMMs sum their partial products in a fixed order)."""
+    n = 30000
+    print("case mixed determinism n=%d" % n, flush=True)
+    pos = W.gaussians(n, 63)
+    den = W.densities(n, 1, 64)
+    hs = []
+    for _ in range(2):
+        f = kifmm.FMM3d(111, [1.0, 0.25], 6, 64, 30, precision="mixed")
+        f.set_points(pos)
+        f.setup()
+        hs.append(f)
+    a = hs[0].evaluate(den)
+    assert np.array_equal(hs[0].evaluate(den), a)
+    assert np.array_equal(hs[1].evaluate(den), a)
+    at = hs[0].evaluate_adjoint(den)
+    assert np.array_equal(hs[1].evaluate_adjoint(den), at)
+
+
+@pytest.mark.parametrize("precision,bar", [("fp64", 0.0), ("mixed", 0.0)])
 def test_graph_replay(gpu, precision, bar):
     """hipGraph replay (kifmm_set_graph) on a non-default stream: three replays
     with new densities copied in between by eager torch work, each against an
-    eager evaluate of the same densities on a second handle.  Bitwise in FP64;
-    the mixed path sums the 8 children of an M2M with FP64 atomics in varying
-    order, which the np-6 pseudo-inverses amplify to ~3e-8 run to run."""
+    eager evaluate of the same densities on a second handle.  Bitwise in both
+    precisions (the mixed path's split-K partial products are summed in chunk
+    order, without atomics)."""
     import torch
     n = 20000
     print("case graph replay %s" % precision, flush=True)
