@@ -401,11 +401,16 @@ __global__ __launch_bounds__(kSurfMx) void k_surface_mx(SurfArgs a, KParams<floa
       }
     }
   }
+  // one-leaf tasks (S2M) are the only writers of their node's check surface:
+  // store; other tasks (S2L) add to what the V list already put there
   double* o = a.out + int64_t(node) * a.sz_out;
   for (int u = 0; u < 2; u++) {
     if (!act[u]) continue;
     const int q = lane + u * kSurfMx;
-    for (int t = 0; t < TD; t++) o[q * TD + t] += acc[u][t];
+    for (int t = 0; t < TD; t++) {
+      if (a.leaf) o[q * TD + t] = acc[u][t];
+      else o[q * TD + t] += acc[u][t];
+    }
   }
 }
 
