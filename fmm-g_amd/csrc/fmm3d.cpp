@@ -673,7 +673,10 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
   // stores it whole (the fused up pass forms no UC for parents)
   if (!mixed && (rc = check(launch_fill(uc_d_.as<double>(), int64_t(N) * op.dat(UC), 0.0, s), "zero UC")))
     return rc;
-  if ((rc = check(launch_fill(de_d_.as<double>(), int64_t(N) * op.dat(DE), 0.0, s), "zero DE"))) return rc;
+  // DE: the downward pseudo-inverse stores it for every evaluator node at depth
+  // >= 2, and only those rows are read (L2L, L2T); the mixed path skips the fill
+  if (!mixed && (rc = check(launch_fill(de_d_.as<double>(), int64_t(N) * op.dat(DE), 0.0, s), "zero DE")))
+    return rc;
   if ((rc = check(launch_fill(dc_d_.as<double>(), int64_t(N) * op.dat(DC), 0.0, s), "zero DC"))) return rc;
   if ((rc = check(launch_gather(d_src_den, sperm_d_.as<int>(), sden_d_.as<double>(), ns_, SD, s), "gather"))) return rc;
 
