@@ -41,6 +41,9 @@ struct NearArgs {
   const double* UE; int sz_ue;
   const double* DE; int sz_de;
   const double* tpos; double* tval;
+  // mixed Laplace near field: if set, value i (leaf order) goes to
+  // tval[tperm[i]], the caller's order (no separate scatter)
+  const int* tperm;
 };
 
 struct SurfArgs {

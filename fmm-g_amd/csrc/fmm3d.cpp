@@ -665,6 +665,7 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
   int rc;
   auto mark = [&](int ph) { if (timing_) (void)hipEventRecord(events_[ph], s); };
   const bool mixed = precision_ == PREC_MIXED;
+  const bool fused_out = mixed && kt_ == 111 && !grad;
 
   mark(PH_INI);
   // zero the work vectors (:425-438)
@@ -771,14 +772,17 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
   if (grad) {
     na.tval = tgrad_d_.as<double>();
     if ((rc = check(launch_near_grad(kt_, na, pu, pm, s), "near field gradient"))) return rc;
-  } else if ((rc = check(mixed ? launch_near_mixed(kt_, na, pu, pm, s) : launch_near(kt_, na, pu, pm, s),
-                         "near field"))) {
-    return rc;
+  } else {
+    // mixed Laplace: the near kernel writes the final values in the caller's order
+    if (fused_out) { na.tval = d_trg_val; na.tperm = tperm_d_.as<int>(); }
+    if ((rc = check(mixed ? launch_near_mixed(kt_, na, pu, pm, s) : launch_near(kt_, na, pu, pm, s), "near field")))
+      return rc;
   }
 
   // write back (:1490-1510)
   mark(PH_FIN);
-  if ((rc = check(launch_scatter(grad ? tgrad_d_.as<double>() : tval_d_.as<double>(), tperm_d_.as<int>(), d_trg_val,
+  if (!fused_out &&
+      (rc = check(launch_scatter(grad ? tgrad_d_.as<double>() : tval_d_.as<double>(), tperm_d_.as<int>(), d_trg_val,
                                  nt_, grad ? 3 : TD, s), "scatter")))
     return rc;
   mark(PH_COUNT);

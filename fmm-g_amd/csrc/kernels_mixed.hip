@@ -322,7 +322,8 @@ __global__ __launch_bounds__(kNearChunk) void k_near_lap_mx(NearArgs a, float pa
       near += red[((u - 1) * tn + lane) * 2];
       far += red[((u - 1) * tn + lane) * 2 + 1];
     }
-    a.tval[int64_t(ch.tbeg + lane)] = near + far;
+    const int64_t o = ch.tbeg + lane;
+    a.tval[a.tperm ? int64_t(a.tperm[o]) : o] = near + far;
   }
 }
 
