@@ -119,7 +119,25 @@ int FMM3d::build_tasks() {
   work_ = Work();
 
   // --- near field: U, W (S2T or M2T), X (S2T fallback), L2T ---
-  near_chunks_h_.clear(); near_items_h_.clear();
+  near_chunks_h_.clear(); near_chunks_mx_h_.clear(); near_items_h_.clear();
+  // Mixed path: the near kernels put S = min(16, 128 / k) copies of a chunk's k
+  // targets on the 128 lanes, so a leaf of 70 targets leaves 58 lanes idle.
+  // Leaves are cut into chunks of sizes that fill the lanes: cost of a chunk =
+  // (active waves) / S sweeps of its sources, plus 0.04 per wave for staging
+  // them again; the cut minimises the sum (dynamic programming over sizes).
+  int max_tn = 0;
+  for (int B = 0; B < N; B++)
+    if ((t.tag[B] & TAG_EVTR) && t.leaf(B)) max_tn = std::max(max_tn, t.trg_num[B]);
+  std::vector<double> split_cost(size_t(max_tn) + 1, 0.0);
+  std::vector<int> split_pick(size_t(max_tn) + 1, 0);
+  for (int n = 1; n <= max_tn; n++) {
+    split_cost[n] = 1e300;
+    for (int k = 1; k <= std::min(n, kNearChunk); k++) {
+      const int S = std::min(16, kNearChunk / k);
+      const double c = double((S * k + 63) / 64) / S + 2 * 0.04 + split_cost[n - k];
+      if (c < split_cost[n] - 1e-12) { split_cost[n] = c; split_pick[n] = k; }
+    }
+  }
   // bounding boxes of every leaf's sources (the near kernel of the mixed path
   // applies the 1e-8 pair cut only where a source can come that close)
   std::vector<double> sbox(size_t(N) * 6);
@@ -166,11 +184,11 @@ int FMM3d::build_tasks() {
       surf += op.n_sam[DE];
     }
     const int iend = int(near_items_h_.size());
-    for (int c = 0; c < tn; c += kNearChunk) {
-      NearChunk ch{B, t.trg_beg[B] + c, std::min(kNearChunk, tn - c), ibeg, iend};
+    auto make_chunk = [&](int c, int k) {
+      NearChunk ch{B, t.trg_beg[B] + c, k, ibeg, iend};
       double tb[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
-      for (int k = 0; k < ch.tnum; k++) {
-        const int64_t o = t.trg_perm[size_t(ch.tbeg) + k];
+      for (int j = 0; j < ch.tnum; j++) {
+        const int64_t o = t.trg_perm[size_t(ch.tbeg) + j];
         for (int d = 0; d < 3; d++) {
           tb[d] = std::min(tb[d], tpos_[3 * o + d]);
           tb[3 + d] = std::max(tb[3 + d], tpos_[3 * o + d]);
@@ -190,7 +208,13 @@ int FMM3d::build_tasks() {
         if (d2 < 1e-14) sep = false;  // within 1e-7: keep the cut on every pair
       }
       ch.unchk = sep ? 1 : 0;
-      near_chunks_h_.push_back(ch);
+      return ch;
+    };
+    for (int c = 0; c < tn; c += kNearChunk) near_chunks_h_.push_back(make_chunk(c, std::min(kNearChunk, tn - c)));
+    for (int c = 0; c < tn;) {
+      const int k = split_pick[tn - c];
+      near_chunks_mx_h_.push_back(make_chunk(c, k));
+      c += k;
     }
     work_.near_exact_pairs += double(tn) * exact;
     work_.near_pairs += double(tn) * (exact + surf);
@@ -441,6 +465,7 @@ int FMM3d::upload_state() {
   if ((rc = check(dc_d_.alloc(sizeof(double) * n * op.dat(DC)), "alloc DC"))) return rc;
   // tasks
   KIFMM_UP(near_chunks_d_, near_chunks_h_, "upload near chunks");
+  KIFMM_UP(near_chunks_mx_d_, near_chunks_mx_h_, "upload near chunks");
   KIFMM_UP(near_items_d_, near_items_h_, "upload near items");
   KIFMM_UP(s2m_tasks_d_, s2m_tasks_h_, "upload S2M tasks");
   KIFMM_UP(s2m_leaf_d_, s2m_leaf_h_, "upload S2M tasks");
@@ -760,6 +785,7 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
   mark(PH_NEAR);
   NearArgs na{};
   na.chunks = near_chunks_d_.as<NearChunk>(); na.nchunks = int(near_chunks_h_.size());
+  if (mixed && !grad) { na.chunks = near_chunks_mx_d_.as<NearChunk>(); na.nchunks = int(near_chunks_mx_h_.size()); }
   na.items = near_items_d_.as<int>();
   na.spos = spos_d_.as<double>(); na.snor = snor_d_.as<double>(); na.sden = sden_d_.as<double>();
   na.src_beg = sbeg_d_.as<int>(); na.src_num = snum_d_.as<int>();

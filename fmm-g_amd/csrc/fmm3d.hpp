@@ -164,6 +164,7 @@ class FMM3d {
   std::vector<int> lvl_idx_h_;
   std::vector<double> scales_h_;
   std::vector<NearChunk> near_chunks_h_;
+  std::vector<NearChunk> near_chunks_mx_h_;  // mixed near field: leaves split for full lanes
   std::vector<int> near_items_h_;
   std::vector<SurfTask> s2m_tasks_h_, s2l_tasks_h_;
   std::vector<int4> s2m_leaf_h_;  // S2M tasks as (leaf, first source, source count)
@@ -181,6 +182,7 @@ class FMM3d {
   DeviceBuffer spos_d_, snor_d_, tpos_d_, sperm_d_, tperm_d_, sden_d_, tval_d_;
   DeviceBuffer geom_d_, depth_d_, sbeg_d_, snum_d_;
   DeviceBuffer ue_d_, uc_d_, dc_d_, de_d_;
+  DeviceBuffer near_chunks_mx_d_;
   DeviceBuffer near_chunks_d_, near_items_d_, s2m_tasks_d_, s2l_tasks_d_, surf_items_d_, s2m_leaf_d_;
   DeviceBuffer lvl_idx_d_;
   DeviceBuffer sam_d_[4], shell_d_, twiddle_d_, deg_d_, ones_d_, scales_d_;
