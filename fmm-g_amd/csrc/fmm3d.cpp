@@ -123,7 +123,7 @@ int FMM3d::build_tasks() {
   // Mixed path: the near kernels put S = min(16, 128 / k) copies of a chunk's k
   // targets on the 128 lanes, so a leaf of 70 targets leaves 58 lanes idle.
   // Leaves are cut into chunks of sizes that fill the lanes: cost of a chunk =
-  // (active waves) / S sweeps of its sources, plus 0.04 per wave for staging
+  // (active waves) / S sweeps of its sources, plus 0.15 per wave for staging
   // them again; the cut minimises the sum (dynamic programming over sizes).
   int max_tn = 0;
   for (int B = 0; B < N; B++)
@@ -134,7 +134,7 @@ int FMM3d::build_tasks() {
     split_cost[n] = 1e300;
     for (int k = 1; k <= std::min(n, kNearChunk); k++) {
       const int S = std::min(16, kNearChunk / k);
-      const double c = double((S * k + 63) / 64) / S + 2 * 0.04 + split_cost[n - k];
+      const double c = double((S * k + 63) / 64) / S + 2 * 0.15 + split_cost[n - k];
       if (c < split_cost[n] - 1e-12) { split_cost[n] = c; split_pick[n] = k; }
     }
   }
