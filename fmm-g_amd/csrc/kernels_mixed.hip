@@ -679,21 +679,25 @@ __global__ __launch_bounds__(GM_T, 2) void k_gemm_mfma(GemmArgs g) {
   kd4 acc[GM_MB];
 #pragma unroll
   for (int mi = 0; mi < GM_MB; mi++) acc[mi] = kd4{0.0, 0.0, 0.0, 0.0};
-  double ra[GM_RA], rb[GM_RB];
-  const double ks0 = g.kscale[0];
+  // the next slab is loaded into registers while this one is multiplied: the
+  // loads are unconditional (an element outside the matrix reads element 0),
+  // and the bounds mask and the K scale are applied when the slab is stored,
+  // so no wait for them sits between the loads and the products
+  double ra[GM_RA], rb[GM_RB], rs[GM_RB];
+  const int64_t bc = bcol[tid >> 3];
   auto fetch = [&](int k0) {
 #pragma unroll
     for (int u = 0; u < GM_RA; u++) {
       const int e = tid + GM_T * u;
       const int m = m0 + e % GM_M, k = k0 + e / GM_M;
-      ra[u] = (m < g.M && k < kend) ? A[int64_t(k) * g.M + m] : 0.0;
+      ra[u] = A[m < g.M && k < kend ? int64_t(k) * g.M + m : 0];
     }
-    const int64_t col = bcol[tid >> 3];
     const int kq0 = k0 + GM_RB * (tid & 7);
 #pragma unroll
     for (int u = 0; u < GM_RB; u++) {
       const int kq = kq0 + u;
-      rb[u] = (col >= 0 && kq < kend) ? g.B[col + kq] * (g.ksd == 1 ? ks0 : g.kscale[kq % g.ksd]) : 0.0;
+      rb[u] = g.B[bc >= 0 && kq < kend ? bc + kq : 0];
+      rs[u] = g.kscale[kq % g.ksd];
     }
   };
   fetch(kbeg);
@@ -701,10 +705,14 @@ __global__ __launch_bounds__(GM_T, 2) void k_gemm_mfma(GemmArgs g) {
 #pragma unroll
     for (int u = 0; u < GM_RA; u++) {
       const int e = tid + GM_T * u;
-      As[e / GM_M][e % GM_M] = ra[u];
+      const int m = m0 + e % GM_M, k = k0 + e / GM_M;
+      As[e / GM_M][e % GM_M] = m < g.M && k < kend ? ra[u] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < GM_RB; u++) Bs[GM_RB * (tid & 7) + u][tid >> 3] = rb[u];
+    for (int u = 0; u < GM_RB; u++) {
+      const int kq = k0 + GM_RB * (tid & 7) + u;
+      Bs[GM_RB * (tid & 7) + u][tid >> 3] = bc >= 0 && kq < kend ? rb[u] * rs[u] : 0.0;
+    }
     __syncthreads();
     if (k0 + GM_K < kend) fetch(k0 + GM_K);
 #pragma unroll
