@@ -351,10 +351,14 @@ __global__ __launch_bounds__(64) void k_fft_fwd(VlistArgs v, C2* dspec) {
     int sh[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
+      // unconditional loads (a lane past the samples reads sample 0), so the
+      // rounds' loads are issued together
       const int q = lane + 64 * r;
       const bool ok = q < NSAM;
-      sh[r] = ok ? v.shell[q] : 0;
-      uv[r] = ok ? v.UE[int64_t(node) * v.sz_ue + q * v.sd + s] * scale : 0.0;
+      const int qq = ok ? q : 0;
+      sh[r] = v.shell[qq];
+      const double u = v.UE[int64_t(node) * v.sz_ue + qq * v.sd + s];
+      uv[r] = ok ? u * scale : 0.0;
     }
     __syncthreads();
     for (int e = lane; e < NP * NP * NP; e += 64) xb[e] = 0.0;
@@ -581,17 +585,24 @@ __global__ __launch_bounds__(64) void k_fft_inv(VlistArgs v, const C2* espec) {
       double2 yy[HH];
 #pragma unroll
       for (int kc = 0; kc < HH; kc++) yy[kc] = load_c(&y2[ab * HH + kc]);
+      // the line's check values are loaded together (absent samples read
+      // element 0 and are not stored), so the loads are not serialised
+      int qc[NP];
+      double old[NP];
 #pragma unroll
       for (int c = 0; c < NP; c++) {
-        const int q = qof[ab * NP + c];
-        if (q < 0) continue;
+        qc[c] = qof[ab * NP + c];
+        old[c] = dc[(qc[c] < 0 ? 0 : qc[c]) * v.td + i];
+      }
+#pragma unroll
+      for (int c = 0; c < NP; c++) {
         double val = yy[0].x + yy[N / 2].x * (((c + O) & 1) ? -1.0 : 1.0);
 #pragma unroll
         for (int kc = 1; kc < N / 2; kc++) {
           const double2 tw = w[((c + O) * kc) % N];
           val += 2.0 * (yy[kc].x * tw.x - yy[kc].y * tw.y);
         }
-        dc[q * v.td + i] += val;
+        if (qc[c] >= 0) dc[qc[c] * v.td + i] = old[c] + val;
       }
     }
   }
