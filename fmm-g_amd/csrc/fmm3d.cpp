@@ -443,6 +443,16 @@ int FMM3d::upload_state() {
   KIFMM_UP(snor_d_, sn, "upload normals");
   KIFMM_UP(tpos_d_, tp, "upload targets");
   KIFMM_UP(sperm_d_, t.src_perm, "upload source permutation");
+  {
+    // the densities are scattered into leaf order: the caller's array is read in
+    // order and each value is written once (random reads of 8 bytes cost a
+    // whole memory burst each)
+    std::vector<int> inv(size_t(std::max<int64_t>(1, ns_)), -1);
+    for (int64_t k = 0; k < ns_; k++) inv[size_t(t.src_perm[size_t(k)])] = int(k);
+    for (int64_t k = 0; k < ns_; k++)
+      if (inv[size_t(k)] < 0) return fail(-6, "source permutation is not a permutation");
+    KIFMM_UP(sinv_d_, inv, "upload source permutation");
+  }
   KIFMM_UP(tperm_d_, t.trg_perm, "upload target permutation");
   if ((rc = check(sden_d_.alloc(sizeof(double) * std::max<int64_t>(1, ns_ * SD)), "alloc densities"))) return rc;
   if ((rc = check(tval_d_.alloc(sizeof(double) * std::max<int64_t>(1, nt_ * TD)), "alloc values"))) return rc;
@@ -704,7 +714,8 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
   if (!mixed && (rc = check(launch_fill(de_d_.as<double>(), int64_t(N) * op.dat(DE), 0.0, s), "zero DE")))
     return rc;
   if ((rc = check(launch_fill(dc_d_.as<double>(), int64_t(N) * op.dat(DC), 0.0, s), "zero DC"))) return rc;
-  if ((rc = check(launch_gather(d_src_den, sperm_d_.as<int>(), sden_d_.as<double>(), ns_, SD, s), "gather"))) return rc;
+  if ((rc = check(launch_scatter(d_src_den, sinv_d_.as<int>(), sden_d_.as<double>(), ns_, SD, s), "load densities")))
+    return rc;
 
   SurfArgs sa{};
   sa.items = surf_items_d_.as<int>();

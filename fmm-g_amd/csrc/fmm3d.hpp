@@ -180,6 +180,7 @@ class FMM3d {
 
   // device state
   DeviceBuffer spos_d_, snor_d_, tpos_d_, sperm_d_, tperm_d_, sden_d_, tval_d_;
+  DeviceBuffer sinv_d_;  // inverse of the source permutation: caller index -> leaf order
   DeviceBuffer geom_d_, depth_d_, sbeg_d_, snum_d_;
   DeviceBuffer ue_d_, uc_d_, dc_d_, de_d_;
   DeviceBuffer near_chunks_mx_d_;
