@@ -493,8 +493,15 @@ __global__ __launch_bounds__(64) void k_hadamard_mx(VlistArgs v, const float2* _
 // Absent halo sources (-1) are skipped; targets outside the V-list target set
 // are computed and dropped.  Frequency chunks are the slow grid dimension and
 // each XCD (workgroup id mod 8) sweeps a contiguous run of tiles.
+//
+// BUF: spectra addressed through buffer resources, the row offset (uniform) in
+// the scalar offset and the lane's frequency in the vector offset, so the
+// loads need no vector address arithmetic (both tables must fit in 32-bit byte
+// offsets).
 constexpr int kTileG = 5;
+typedef float tf2 __attribute__((ext_vector_type(2)));
 
+template <bool BUF>
 __global__ __launch_bounds__(64) void k_m2l_tile(VlistArgs v, const float2* __restrict__ dspec,
                                                  const float2* __restrict__ m2l, float2* __restrict__ espec,
                                                  float nrm, int ntile, int per_xcd) {
@@ -508,6 +515,28 @@ __global__ __launch_bounds__(64) void k_m2l_tile(VlistArgs v, const float2* __re
   const int fs = fok ? f : H - 1;
   const int g0 = tile * G;
   const int ng = min(G, v.ngrp - g0);
+  const unsigned H8 = unsigned(H) * 8u, fs8 = unsigned(fs) * 8u;
+  // (unused when !BUF)
+  const __amdgpu_buffer_rsrc_t rs_x =
+      __builtin_amdgcn_make_buffer_rsrc((void*)dspec, (short)0, int(unsigned(v.nsrc) * H8), 0x00020000);
+  const __amdgpu_buffer_rsrc_t rs_t =
+      __builtin_amdgcn_make_buffer_rsrc((void*)m2l, (short)0, int(343u * H8), 0x00020000);
+  auto ldx = [&](int row) -> float2 {
+    if constexpr (BUF) {
+      const tf2 x = __builtin_bit_cast(tf2, __builtin_amdgcn_raw_buffer_load_b64(rs_x, fs8, unsigned(row) * H8, 0));
+      return make_float2(x.x, x.y);
+    } else {
+      return dspec[int64_t(row) * H + fs];
+    }
+  };
+  auto ldt = [&](int slot) -> float2 {
+    if constexpr (BUF) {
+      const tf2 t = __builtin_bit_cast(tf2, __builtin_amdgcn_raw_buffer_load_b64(rs_t, fs8, unsigned(slot) * H8, 0));
+      return make_float2(t.x, t.y);
+    } else {
+      return m2l[int64_t(slot) * H + fs];
+    }
+  };
   __shared__ int shal[G * 216];
   for (int i = threadIdx.x; i < G * 216; i += 64) shal[i] = i < ng * 216 ? v.halo[int64_t(g0) * 216 + i] : -1;
   __syncthreads();
@@ -528,7 +557,7 @@ __global__ __launch_bounds__(64) void k_m2l_tile(VlistArgs v, const float2* __re
     for (int c = 0; c < 8; c++)
       si[c] = __builtin_amdgcn_readfirstlane(hal[(c >> 2) * 36 + ((c >> 1) & 1) * 6 + (c & 1)]);
 #pragma unroll
-    for (int c = 0; c < 8; c++) x[c] = dspec[int64_t(si[c] < 0 ? 0 : si[c]) * H + fs];
+    for (int c = 0; c < 8; c++) x[c] = ldx(si[c] < 0 ? 0 : si[c]);
   };
   int sc[8];
   float2 xc[8];
@@ -541,7 +570,7 @@ __global__ __launch_bounds__(64) void k_m2l_tile(VlistArgs v, const float2* __re
 #pragma unroll
     for (int d = 0; d < 27; d++) {
       const int sl = sbase + (d / 9 - 1) * 49 + ((d / 3) % 3 - 1) * 7 + (d % 3 - 1);
-      const float2 t = m2l[int64_t(sl) * H + fs];
+      const float2 t = ldt(sl);
       ta[d] = t.x;
       tb[d] = t.y;
       ts[d] = t.x + t.y;
@@ -625,8 +654,11 @@ hipError_t launch_hadamard_mixed(const VlistArgs& v, const float2* dspec, const 
   if (v.sd == 1 && v.td == 1) {
     const int ntile = (v.ngrp + kTileG - 1) / kTileG;
     const int per_xcd = (ntile + 7) / 8;
-    k_m2l_tile<<<dim3(unsigned(8 * per_xcd), unsigned((v.H + 63) / 64)), dim3(64), 0, s>>>(
-        v, dspec, m2l, espec, nrm, ntile, per_xcd);
+    const dim3 grid2(unsigned(8 * per_xcd), unsigned((v.H + 63) / 64));
+    if (uint64_t(v.nsrc) * v.H * 8 <= 0xFFFFFF00ull)
+      k_m2l_tile<true><<<grid2, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm, ntile, per_xcd);
+    else
+      k_m2l_tile<false><<<grid2, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm, ntile, per_xcd);
   }
   else if (v.sd == 4 && v.td == 3) k_hadamard_mx<4, 3><<<grid, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm);
   else if (v.sd == 3 && v.td == 3) k_hadamard_mx<3, 3><<<grid, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm);
