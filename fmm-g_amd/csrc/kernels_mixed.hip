@@ -687,9 +687,8 @@ constexpr int GM_M = 160, GM_N = 64, GM_K = 16, GM_T = 512, GM_MB = GM_M / 32;
 constexpr int GM_RA = GM_M * GM_K / GM_T, GM_RB = GM_K * GM_N / GM_T;
 
 __global__ __launch_bounds__(GM_T, 2) void k_gemm_mfma(GemmArgs g) {
-  __shared__ double As[GM_K][GM_M + 16];
-  __shared__ double Bs[GM_K][GM_N + 16];
-  __shared__ int64_t bcol[GM_N];
+  __shared__ double As[2][GM_K][GM_M + 16];  // two slabs: one multiplied, the next stored
+  __shared__ double Bs[2][GM_K][GM_N + 16];
   const bool split = g.ksplit > 1;
   const int zk = split ? int(blockIdx.z) % g.ksplit : 0;
   const int z = split ? int(blockIdx.z) / g.ksplit : int(blockIdx.z);
@@ -703,11 +702,6 @@ __global__ __launch_bounds__(GM_T, 2) void k_gemm_mfma(GemmArgs g) {
   const int m0 = blockIdx.x * GM_M;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
   const int wc = wave & 3, wr = wave >> 2;  // column block, row half
-  if (tid < GM_N) {
-    const int j = n0 + tid;
-    bcol[tid] = j < cend ? int64_t(g.bidx[j]) * g.ldb : int64_t(-1);
-  }
-  __syncthreads();
   kd4 acc[GM_MB];
 #pragma unroll
   for (int mi = 0; mi < GM_MB; mi++) acc[mi] = kd4{0.0, 0.0, 0.0, 0.0};
@@ -716,7 +710,9 @@ __global__ __launch_bounds__(GM_T, 2) void k_gemm_mfma(GemmArgs g) {
   // and the bounds mask and the K scale are applied when the slab is stored,
   // so no wait for them sits between the loads and the products
   double ra[GM_RA], rb[GM_RB], rs[GM_RB];
-  const int64_t bc = bcol[tid >> 3];
+  // the B column this thread stages (8 threads a column)
+  const int jb = n0 + (tid >> 3);
+  const int64_t bc = jb < cend ? int64_t(g.bidx[jb]) * g.ldb : int64_t(-1);
   auto fetch = [&](int k0) {
 #pragma unroll
     for (int u = 0; u < GM_RA; u++) {
@@ -732,29 +728,40 @@ __global__ __launch_bounds__(GM_T, 2) void k_gemm_mfma(GemmArgs g) {
       rs[u] = g.kscale[kq % g.ksd];
     }
   };
-  fetch(kbeg);
-  for (int k0 = kbeg; k0 < kend; k0 += GM_K) {
+  auto store = [&](int buf, int k0) {
 #pragma unroll
     for (int u = 0; u < GM_RA; u++) {
       const int e = tid + GM_T * u;
       const int m = m0 + e % GM_M, k = k0 + e / GM_M;
-      As[e / GM_M][e % GM_M] = m < g.M && k < kend ? ra[u] : 0.0;
+      As[buf][e / GM_M][e % GM_M] = m < g.M && k < kend ? ra[u] : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < GM_RB; u++) {
       const int kq = k0 + GM_RB * (tid & 7) + u;
-      Bs[GM_RB * (tid & 7) + u][tid >> 3] = bc >= 0 && kq < kend ? rb[u] * rs[u] : 0.0;
+      Bs[buf][GM_RB * (tid & 7) + u][tid >> 3] = bc >= 0 && kq < kend ? rb[u] * rs[u] : 0.0;
     }
-    __syncthreads();
-    if (k0 + GM_K < kend) fetch(k0 + GM_K);
+  };
+  fetch(kbeg);
+  store(0, kbeg);
+  __syncthreads();
+  // one barrier a slab: the store into the other buffer follows this slab's
+  // products, and every wave has finished reading that buffer (two slabs back)
+  // before the previous barrier
+  int buf = 0;
+  for (int k0 = kbeg; k0 < kend; k0 += GM_K) {
+    const bool more = k0 + GM_K < kend;
+    if (more) fetch(k0 + GM_K);
 #pragma unroll
     for (int ks = 0; ks < GM_K / 4; ks++) {
-      const double b = Bs[4 * ks + lk][16 * wc + li];
+      const double b = Bs[buf][4 * ks + lk][16 * wc + li];
 #pragma unroll
       for (int mi = 0; mi < GM_MB; mi++)
-        acc[mi] = __builtin_amdgcn_mfma_f64_16x16x4f64(As[4 * ks + lk][80 * wr + 16 * mi + li], b, acc[mi], 0, 0, 0);
+        acc[mi] = __builtin_amdgcn_mfma_f64_16x16x4f64(As[buf][4 * ks + lk][80 * wr + 16 * mi + li], b, acc[mi], 0, 0,
+                                                       0);
     }
+    if (more) store(buf ^ 1, k0 + GM_K);
     __syncthreads();
+    buf ^= 1;
   }
   const int n = n0 + 16 * wc + li;
   if (n >= cend) return;
