@@ -484,11 +484,12 @@ __global__ __launch_bounds__(64) void k_hadamard_mx(VlistArgs v, const float2* _
 //   ngrp * 864 bytes.
 // * The source spectra of the next (q, group) are loaded while the current
 //   group's multiply-adds run.
-// * Three real accumulators per complex output (Karatsuba).  With T = a + ib
-//   and X = c + id: K1 += c (a + b), K2 += a (d - c), K3 += b (c + d), so
-//   Re = K1 - K3 = sum (ac - bd) and Im = K1 + K2 = sum (ad + bc).  a + b is
-//   formed once per translation spectrum and q, d - c and c + d once per source
-//   spectrum and step: 3 FMAs per complex multiply-add instead of 4.
+// * Three real accumulators per complex output (Gauss's three-product form).
+//   With T = a + ib and X = c + id: A1 += a c, A2 += b d, A3 += (a + b)(c + d),
+//   so Re = A1 - A2 = sum (ac - bd) and Im = A3 - A1 - A2 = sum (ad + bc).
+//   a + b is formed once per translation spectrum and q, c + d once per source
+//   spectrum and step: 3 FMAs per complex multiply-add instead of 4, and one
+//   extra add per source.
 //
 // Absent halo sources (-1) are skipped; targets outside the V-list target set
 // are computed and dropped.  Frequency chunks are the slow grid dimension and
@@ -584,14 +585,14 @@ __global__ __launch_bounds__(64) void k_m2l_tile(VlistArgs v, const float2* __re
 #pragma unroll
       for (int c = 0; c < 8; c++) {
         if (sc[c] < 0) continue;
-        const float xr = xc[c].x, xm = xc[c].y - xc[c].x, xs = xc[c].x + xc[c].y;
+        const float xr = xc[c].x, xi = xc[c].y, xs = xc[c].x + xc[c].y;
 #pragma unroll
         for (int t = 0; t < 8; t++) {
           const int d = ((c >> 2) - (t >> 2) + 1) * 9 + (((c >> 1) & 1) - ((t >> 1) & 1) + 1) * 3 +
                         ((c & 1) - (t & 1) + 1);
-          k1[gi][t] = fmaf(xr, ts[d], k1[gi][t]);
-          k2[gi][t] = fmaf(xm, ta[d], k2[gi][t]);
-          k3[gi][t] = fmaf(xs, tb[d], k3[gi][t]);
+          k1[gi][t] = fmaf(xr, ta[d], k1[gi][t]);
+          k2[gi][t] = fmaf(xi, tb[d], k2[gi][t]);
+          k3[gi][t] = fmaf(xs, ts[d], k3[gi][t]);
         }
       }
       if (qn < 27) {
@@ -608,7 +609,8 @@ __global__ __launch_bounds__(64) void k_m2l_tile(VlistArgs v, const float2* __re
     for (int t = 0; t < 8; t++) {
       const int tg = v.tgt[(g0 + gi) * 8 + t];
       if (tg >= 0)
-        espec[int64_t(tg) * H + f] = make_float2((k1[gi][t] - k3[gi][t]) * nrm, (k1[gi][t] + k2[gi][t]) * nrm);
+        espec[int64_t(tg) * H + f] =
+            make_float2((k1[gi][t] - k2[gi][t]) * nrm, (k3[gi][t] - k1[gi][t] - k2[gi][t]) * nrm);
     }
   }
 }
