@@ -207,6 +207,32 @@ def test_mixed_near_cut_across_leaves():
     assert rel(out["mixed"][k], out["fp64"][k]) < 1e-5
 
 
+def test_mixed_leaf_split():
+    """The mixed near field cuts each leaf's targets into chunks sized to fill
+    the lanes (any size up to 128, several chunks per leaf).  Clustered points
+    with ptsMax 128 and maxLevel 4 give leaves of every size, 22 of them beyond
+    128 targets (up to 5846); every target must still get its whole near field
+    once.  In leaves that large the FP32 coordinates of close pairs put the
+    mixed result 6.4e-6 from FP64 (the same with one chunk per 128 targets),
+    so the bar is 2e-5: a target missed or counted twice moves it by ~1e-2."""
+    n = 30000
+    pos = W.gaussians(n, 98)
+    den = W.densities(n, 1, 99, kind="normal")
+    print("case mixed leaf split n=%d" % n, flush=True)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        f = kifmm.FMM3d(111, [1.0, 0.25], 4, 128, 4, precision=prec)
+        f.set_points(pos)
+        f.setup()
+        out[prec] = f.evaluate(den)
+        out[prec + "_err"] = f.check(den, out[prec], 2000, seed=11)
+    d = rel(out["mixed"], out["fp64"])
+    print("leaf split: mixed vs fp64 %.3e, errors vs direct: fp64 %.3e mixed %.3e"
+          % (d, out["fp64_err"], out["mixed_err"]), flush=True)
+    assert d < 2e-5
+    assert out["mixed_err"] <= out["fp64_err"] + 2 * d
+
+
 def test_batched_problems_multi_rhs():
     """The many-problem workload, reduced: independent Navier problems (np 8,
     ptsMax 120, maxLevel 10, nu 0.3), each with 3 right-hand sides through
