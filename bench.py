@@ -1,15 +1,25 @@
 """Benchmark: one warm KIFMM3d evaluate on one MI355X.
 
-Workload (default): 8,388,608 nonuniform points (90% from 16 Gaussians, 10%
+Headline workload: 8,388,608 nonuniform points (90% from 16 Gaussians, 10%
 uniform), Laplace single layer, np 6, ptsMax 128, maxLevel 14; error against
 the FP64 direct sum at 4,096 sampled targets.  Points, tree, lists, operators
 and densities are resident in HBM before the timed region; one step is one
-evaluate (device pointers in, device pointers out).
+evaluate (device pointers in, device pointers out).  The mixed-precision and
+the plain FP64 paths are both timed, and the line carries a roofline fraction
+for every phase (reference PETSc event names) plus the whole evaluation.
+
+Beside the headline (extra keys of the same JSON line): the same workload at
+np 7, the smallest np whose discretisation reaches 1e-6; the 32,768-point
+uniform case (FP64); the Stokes double layer on 32 spheres at 1,048,576 points
+(FP64 and mixed); the 1,048,576-point gradient workload (forward, transposed
+and field passes).
 
 Prints one JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
 """
 import argparse
+import gc
 import json
+import math
 import os
 import sys
 import time
@@ -20,18 +30,27 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "fmm-g_amd"))
 
 METRIC = "Mpoints/s per FMM eval at ≤1e-6 rel err, mixed vs FP64 GPU; % roofline"
-HBM_PEAK_GBS = 8000.0
-PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+HBM_GBS = 8000.0
+PEAK_TF = {"fp64": 78.6, "fp32": 157.3}
 
+# name: kernel, coefs, np, ptsMax, maxLevel, generator, n, config index (seed stream)
 CONFIGS = {
-    # name: (kernel, coefs, np, ptsMax, maxLevel, generator, n)
-    "c1": (111, [1.0, 0.25], 4, 64, 30, "uniform", 32768),
-    "c3": (111, [1.0, 0.25], 6, 128, 14, "gaussians", 8388608),
+    "c1": (111, [1.0, 0.25], 4, 64, 30, "uniform", 32768, 0),
+    "c2": (321, [1.0, 0.25], 6, 64, 30, "spheres", 1048576, 1),
+    "c3": (111, [1.0, 0.25], 6, 128, 14, "gaussians", 8388608, 2),
+    "c4": (111, [1.0, 0.25], 6, 64, 30, "uniform", 1048576, 3),
 }
 WORKLOAD_NAME = {
     "c1": "Laplace-S uniform N=32768 np=4 ptsMax=64",
+    "c2": "Stokes-D (321) on 32 spheres N=1048576 np=6 ptsMax=64",
     "c3": "Laplace-S nonuniform (16 Gaussians + 10% uniform) N=8388608 np=6 ptsMax=128 maxLevel=14",
+    "c4": "Laplace-S uniform N=1048576 np=6 ptsMax=64 (forward, transposed and field passes)",
 }
+# flops per (target, source) pair of the user kernel: Laplace single layer 11
+# (3 sub, 5 for r^2, 1 rsqrt, 2 FMA); Stokes double layer 28 (r^2, rsqrt, r^-5,
+# r.n, then 9 products and FMAs into 3 outputs per 3-vector density)
+PAIR_FLOPS = {111: 11.0, 321: 28.0, 511: 27.0}
+MM = {111: (111, 1, 1), 321: (301, 4, 3), 511: (511, 3, 3)}
 
 
 def make_points(gen, n, cfg_index):
@@ -40,6 +59,8 @@ def make_points(gen, n, cfg_index):
         return W.uniform(n, W.seed(cfg_index, 0)), None
     if gen == "gaussians":
         return W.gaussians(n, W.seed(cfg_index, 0)), None
+    if gen == "spheres":
+        return W.spheres(n, W.seed(cfg_index, 0), nsph=32)
     raise ValueError(gen)
 
 
@@ -59,22 +80,210 @@ def traffic_bytes(kernel):
     return int((k["fetch_kib"] * d["fetch_scale"] + k["write_kib"] * d["write_scale"]) * 1024)
 
 
-def cpu_baseline(cfg, precision_ops, n_sample, cfg_index):
+def host_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"cpu_model": model, "nproc": os.cpu_count(), "nproc_available": avail}
+
+
+def cpu_baseline(cfg, pinv, n_sample):
     """The CPU oracle (reference algorithm restated in C, 1 thread) on a bounded
-    sample of the same distribution; returns Mpoints/s and the sample size."""
+    sample of the same distribution: the tree and operators are built first,
+    one cold evaluate, then the timed warm second evaluate (tt0.cpp:501-524)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from kifmm import workloads as W
-    kt, coefs, np_, pm, ml, gen, _ = cfg
-    pos, _ = make_points(gen, n_sample, cfg_index)
-    den = W.densities(n_sample, 1, W.seed(cfg_index, 1))
-    pu, pd = precision_ops
+    kt, coefs, np_, pm, ml, gen, _, ci = cfg
+    pos, nor = make_points(gen, n_sample, ci)
+    nor = np.zeros_like(pos) if nor is None else nor
+    den = W.densities(n_sample, 1, W.seed(ci, 1))
+    pu, pd = pinv
     ops = O.Ops(O.mm_kernel(kt), coefs, np_, pu, pd)
     tree = O.Tree(pos, pts_max=pm, max_level=ml)
     t0 = time.perf_counter()
-    O.evaluate(tree, ops, kt, coefs, pos, np.zeros_like(pos), pos, den)
-    dt = time.perf_counter() - t0
-    return n_sample / dt / 1e6, dt
+    O.evaluate(tree, ops, kt, coefs, pos, nor, pos, den)
+    cold = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.evaluate(tree, ops, kt, coefs, pos, nor, pos, den)
+    warm = time.perf_counter() - t0
+    return n_sample / warm / 1e6, warm, cold
+
+
+def phase_roofline(kt, np_, w, phases, mixed, n_src, n_trg):
+    """Per-phase roofline: T_roof = max(F / peak, B / HBM) from the work counts of
+    the built tree, fraction = T_roof / T_measured, and sum(T_roof) / T_eval.
+    F and B are algorithmic: pair flops at c_K per pair (S2T, S2M, S2L, M2T,
+    L2T); 2 M K per GEMM column; 2.5 G log2 G per FFT line set of a box and
+    DOF; 8 flops per complex multiply-add of the Hadamard.  B counts every
+    operand once.  Peaks: FP32 157.3 TFLOP/s (mixed near field, surfaces,
+    FFTs, Hadamard), FP64 78.6 TFLOP/s (GEMMs; everything in the FP64 path),
+    HBM 8 TB/s."""
+    km, sd, td = MM[kt]
+    SD = {111: 1, 321: 3, 511: 3}[kt]
+    TD = {111: 1, 321: 3, 511: 3}[kt]
+    cK = PAIR_FLOPS[kt]
+    nue = np_ ** 3 - (np_ - 2) ** 3
+    nuc = (np_ + 2) ** 3 - np_ ** 3
+    ue, uc, de, dc = nue * sd, nuc * td, nue * sd, nue * td
+    G = (2 * np_) ** 3
+    H = (np_ + 1) * (2 * np_) ** 2
+    vec = PEAK_TF["fp32" if mixed else "fp64"] * 1e12
+    f64 = PEAK_TF["fp64"] * 1e12
+    bw = HBM_GBS * 1e9
+    cw = 8 if mixed else 16  # bytes per complex spectrum entry
+
+    def troof(parts):
+        return sum(max(F / P, B / bw) for F, P, B in parts)
+
+    parents, contrib = w["m2m_cols"], w["up_cols"]
+    leaves = contrib - parents
+    if mixed:
+        up_gemm = [(2.0 * ue * 8 * ue * parents, f64, 8.0 * (ue * 8 * ue + parents * (8 * ue + ue))),
+                   (2.0 * ue * uc * leaves, f64, 8.0 * (ue * uc + leaves * (uc + ue)))]
+    else:
+        up_gemm = [(2.0 * uc * 8 * ue * parents, f64, 8.0 * (uc * 8 * ue + parents * (8 * ue + uc))),
+                   (2.0 * ue * uc * contrib, f64, 8.0 * (ue * uc + contrib * (uc + ue)))]
+    s2m = [(cK * w["s2m_pairs"], vec, 8.0 * (n_src * (3 + SD)) + 8.0 * leaves * uc)]
+    fft = [(2.5 * G * math.log2(G) * (sd * w["n_vsrc"] + td * w["n_vtrg"]), vec,
+            8.0 * ue * w["n_vsrc"] + cw * H * (sd * w["n_vsrc"] + td * w["n_vtrg"]) + 16.0 * dc * w["n_vtrg"])]
+    had = [(8.0 * H * sd * td * w["vlist_pairs"], vec,
+            cw * H * (sd * w["n_vsrc"] + td * w["n_vtrg"]) + cw * 316 * sd * td * H)]
+    s2l = [(cK * w["s2l_pairs"], vec, 8.0 * n_src * (3 + SD))]
+    down = [(2.0 * dc * de * w["l2l_cols"], f64, 8.0 * (8 * dc * de + w["l2l_cols"] * (de + dc))),
+            (2.0 * de * dc * w["dn_cols"], f64, 8.0 * (de * dc + w["dn_cols"] * (dc + de)))]
+    near = [(cK * w["near_pairs"], vec, 8.0 * (n_src * (3 + SD) + n_trg * (3 + TD)))]
+    # density load into leaf order and the write-back to the caller's order:
+    # each value read once and written once
+    ini = [(0.0, vec, 16.0 * n_src * SD)]
+    # (the mixed Laplace near kernel writes the caller's order itself)
+    fin = [] if (mixed and kt == 111) else [(0.0, vec, 16.0 * n_trg * TD)]
+    out = {}
+    rows = {"EvlIni": ini, "EvlUpwComp": s2m + up_gemm, "EvlVList": fft + had, "EvlXList": s2l,
+            "EvlCombine": down, "EvlUList": near, "EvlFinalize": fin}
+    total_roof = 0.0
+    for name, parts in rows.items():
+        t = troof(parts)
+        total_roof += t
+        meas = phases.get(name, 0.0) * 1e-3
+        out[name] = {"t_roof_ms": round(t * 1e3, 4), "t_ms": round(meas * 1e3, 4),
+                     "frac": round(t / meas, 4) if meas > 0 else None}
+    out["VListHadamard"] = {"t_roof_ms": round(troof(had) * 1e3, 4),
+                            "t_ms": round(phases.get("VListHadamard", 0.0), 4),
+                            "frac": round(troof(had) / (phases["VListHadamard"] * 1e-3), 4)
+                            if phases.get("VListHadamard") else None}
+    return out, total_roof
+
+
+class Runner:
+    def __init__(self, dev, steps, warmup, numchk):
+        self.dev, self.steps, self.warmup, self.numchk = dev, steps, warmup, numchk
+
+    def inputs(self, cfg, n):
+        from kifmm import workloads as W
+        import torch
+        kt, coefs, np_, pm, ml, gen, _, ci = cfg
+        pos, nor = make_points(gen, n, ci)
+        sd = {111: 1, 321: 3, 511: 3}[kt]
+        den = W.densities(n, sd, W.seed(ci, 1), kind="uniform" if sd == 1 else "normal")
+        rng = np.random.default_rng(W.seed(ci, 3))
+        idx = rng.choice(n, size=min(self.numchk, n), replace=False)
+        return pos, nor, den, torch.from_numpy(den).to(self.dev), idx
+
+    def run(self, cfg, n, precision, pos, nor, d_den, idx, steps=None, warmup=None, np_override=None, tag=""):
+        """Setup, warm-up, K timed evaluates (plain stream, then a graph replay
+        on a stream of its own), a few evaluates with phase timing, error vs
+        the FP64 direct sum at the sampled targets."""
+        import torch
+        import kifmm
+        kt, coefs, np_, pm, ml, _, _, _ = cfg
+        if np_override:
+            np_ = np_override
+        steps = steps or self.steps
+        warmup = self.warmup if warmup is None else warmup
+        t0 = time.perf_counter()
+        f = kifmm.FMM3d(kt, coefs, np_, pm, ml, precision=precision)
+        f.set_points(pos, nor)
+        f.setup()
+        setup_s = time.perf_counter() - t0
+        d_val = torch.empty(n * f.trg_dof, dtype=torch.float64, device=self.dev)
+        torch.cuda.synchronize()
+        for _ in range(warmup):
+            f.evaluate_device(d_den, d_val)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            f.evaluate_device(d_den, d_val)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        # the same pass replayed from a captured hipGraph on a stream of its own
+        st = torch.cuda.Stream(device=self.dev)
+        f.set_graph(True)
+        with torch.cuda.stream(st):
+            f.evaluate_device(d_den, d_val, stream=st.cuda_stream)
+            st.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                f.evaluate_device(d_den, d_val, stream=st.cuda_stream)
+            st.synchronize()
+            dt_graph = (time.perf_counter() - t0) / steps
+        f.set_graph(False)
+        torch.cuda.synchronize()
+        # phase breakdown: HIP events on the evaluate stream (timing mode
+        # synchronises after each evaluate)
+        f.set_timing(True)
+        nph = max(1, min(steps, 5))
+        phase_sum = {}
+        for _ in range(nph):
+            f.evaluate_device(d_den, d_val)
+            for k, v in f.phase_times().items():
+                phase_sum[k] = phase_sum.get(k, 0.0) + v
+        f.set_timing(False)
+        phases = {k: v / nph for k, v in phase_sum.items()}
+        torch.cuda.synchronize()
+        d_chk = torch.from_numpy(np.ascontiguousarray(pos[idx])).to(self.dev)
+        d_ref = torch.empty(len(idx) * f.trg_dof, dtype=torch.float64, device=self.dev)
+        t0 = time.perf_counter()
+        f.direct_device(d_den, d_chk, d_ref)
+        torch.cuda.synchronize()
+        direct_ms = (time.perf_counter() - t0) * 1e3
+        ref = d_ref.cpu().numpy().reshape(len(idx), -1)
+        val = d_val.cpu().numpy()
+        td = f.trg_dof
+        got = val.reshape(-1, td)[idx]
+        rel_err = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+        print("[%s %s np=%d] setup %.2fs  ms/step %.3f (graph %.3f)  rel_err %.3e  direct %.1f ms  phases %s"
+              % (tag, precision, np_, setup_s, dt * 1e3, dt_graph * 1e3, rel_err, direct_ms,
+                 json.dumps({k: round(v, 3) for k, v in phases.items()})), file=sys.stderr, flush=True)
+        return {"f": f, "dt": dt, "dt_graph": dt_graph, "phases": phases, "rel_err": rel_err, "val": val,
+                "setup_s": setup_s, "direct_ms": direct_ms}
+
+
+def pair_summary(mx, f64, n):
+    d = float(np.linalg.norm(mx["val"] - f64["val"]) / np.linalg.norm(f64["val"]))
+    return {"mixed_ms": round(mx["dt"] * 1e3, 4), "fp64_ms": round(f64["dt"] * 1e3, 4),
+            "mixed_graph_ms": round(mx["dt_graph"] * 1e3, 4),
+            "mixed_mpoints_s": round(n / mx["dt"] / 1e6, 3), "fp64_mpoints_s": round(n / f64["dt"] / 1e6, 3),
+            "speedup_vs_fp64": round(f64["dt"] / mx["dt"], 3),
+            "rel_err_mixed": mx["rel_err"], "rel_err_fp64": f64["rel_err"], "rel_diff_vs_fp64": d}
+
+
+def release(*rs):
+    for r in rs:
+        if r and "f" in r:
+            r["f"].close()
+            r.pop("f", None)
+    gc.collect()
 
 
 def main():
@@ -82,11 +291,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--no-extra", action="store_true", help="headline workload only")
     ap.add_argument("--n", type=int, default=0, help="override the point count (diagnostics only)")
-    ap.add_argument("--cpu-sample", type=int, default=131072)
+    ap.add_argument("--cpu-sample", type=int, default=262144)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--numchk", type=int, default=4096)
     ap.add_argument("--calib-copy", action="store_true",
@@ -98,101 +307,57 @@ def main():
         sys.exit(2)
 
     import torch
-    import kifmm
-    from kifmm import workloads as W
 
-    cfg = CONFIGS[args.config]
-    cfg_index = {"c1": 0, "c3": 2}[args.config]
-    kt, coefs, np_, pm, ml, gen, n = cfg
-    if args.n:
-        n = args.n
     dev = torch.device("cuda:0")
-    pos, nor = make_points(gen, n, cfg_index)
-    den = W.densities(n, 1, W.seed(cfg_index, 1))
-    d_den = torch.from_numpy(den).to(dev)
     if args.calib_copy:
         src = torch.ones(64 * 2 ** 20, dtype=torch.float64, device=dev)  # 512 MiB
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    rng = np.random.default_rng(W.seed(cfg_index, 3))
-    idx = rng.choice(n, size=min(args.numchk, n), replace=False)
+    R = Runner(dev, args.steps, args.warmup, args.numchk)
 
-    def run(precision):
-        """Setup, warm-up, K timed evaluates; returns timings, phases and error."""
-        t0 = time.perf_counter()
-        f = kifmm.FMM3d(kt, coefs, np_, pm, ml, precision=precision)
-        f.set_points(pos, nor)
-        f.setup()
-        print("[%s] setup %.2fs n=%d" % (precision, time.perf_counter() - t0, n), file=sys.stderr, flush=True)
-        d_val = torch.empty(n * f.trg_dof, dtype=torch.float64, device=dev)
-        torch.cuda.synchronize()
-        for _ in range(args.warmup):
-            f.evaluate_device(d_den, d_val)
-        torch.cuda.synchronize()
-        f.set_timing(True)
-        phase_sum = {}
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            f.evaluate_device(d_den, d_val)
-            for k, v in f.phase_times().items():
-                phase_sum[k] = phase_sum.get(k, 0.0) + v
-        torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / args.steps
-        f.set_timing(False)
-        phases = {k: v / args.steps for k, v in phase_sum.items()}
-        # accuracy vs the FP64 direct sum at sampled targets (fmm3d_check_mpi.cpp:159-165)
-        d_chk = torch.from_numpy(np.ascontiguousarray(pos[idx])).to(dev)
-        d_ref = torch.empty(len(idx) * f.trg_dof, dtype=torch.float64, device=dev)
-        f.direct_device(d_den, d_chk, d_ref)
-        torch.cuda.synchronize()
-        ref = d_ref.cpu().numpy()
-        val = d_val.cpu().numpy()
-        rel_err = float(np.linalg.norm(val[idx] - ref) / np.linalg.norm(ref))
-        print("[%s] ms/step %.3f rel_err %.3e phases %s" % (precision, dt * 1e3, rel_err,
-              json.dumps({k: round(v, 3) for k, v in phases.items()})), file=sys.stderr, flush=True)
-        return f, dt, phases, rel_err, val
-
-    f64 = dt64 = None
+    # ---- headline: the 8M-point clustered Laplace workload, np 6 ----
+    cfg = CONFIGS["c3"]
+    kt, coefs, np_, pm, ml, gen, n, ci = cfg
+    if args.n:
+        n = args.n
+    pos, nor, den, d_den, idx = R.inputs(cfg, n)
+    r64 = None
     if args.precision == "mixed" and not args.no_fp64:
-        f64, dt64, ph64, err64, val64 = run("fp64")
-        del f64
-    f, dt, phases, rel_err, val = run(args.precision)
-    diff_fp64 = None
-    if dt64 is not None:
-        diff_fp64 = float(np.linalg.norm(val - val64) / np.linalg.norm(val64))
+        r64 = R.run(cfg, n, "fp64", pos, nor, d_den, idx, tag="c3")
+        r64["f"].close()
+    r = R.run(cfg, n, args.precision, pos, nor, d_den, idx, tag="c3")
+    f = r["f"]
+    mixed = args.precision == "mixed"
+    w = f.work_counts()
+    phases = r["phases"]
+    per_phase, total_roof = phase_roofline(kt, np_, w, phases, mixed, n, n)
 
-    # roofline of the dominant kernel: the V-list Hadamard (k_m2l_tile in the
+    # roofline of the dominant kernel, the V-list Hadamard (k_m2l_tile in the
     # mixed path, k_hadamard in FP64).  Algorithmic work: one complex
     # multiply-add (8 flops) per V-list (target, source) pair, frequency and
     # sd*td component.  Compulsory bytes: every source and target spectrum once
     # and the 316 translation spectra, (n_vsrc*sd + n_vtrg*td)*H*2w +
-    # 316*sd*td*H*2w with w = 4 (FP32) or 8 (FP64).  The intensity (about 78
+    # 316*sd*td*H*2w with w = 4 (FP32) or 8 (FP64).  The intensity (about 76
     # flop/byte in FP32) is far above the machine balance (157.3 TFLOP/s over
-    # 8 TB/s, about 20), so the compute peak is the bound.
-    w = f.work_counts()
-    mixed = args.precision == "mixed"
+    # 8 TB/s, about 20): the kernel is bound by its vector (VALU) issue, and
+    # the chip's FP32 peak is the roof.
     had_ms = phases["VListHadamard"]
     wbytes = 4 if mixed else 8
     H = (np_ + 1) * (2 * np_) ** 2
-    sd = td = 1  # equivalent-density kernel 111
-    alg_bytes = (w["n_vsrc"] * sd + w["n_vtrg"] * td) * H * 2 * wbytes + 316 * sd * td * H * 2 * wbytes
-    alg_flops = 8.0 * w["vlist_pairs"] * H * sd * td
+    alg_bytes = (w["n_vsrc"] + w["n_vtrg"]) * H * 2 * wbytes + 316 * H * 2 * wbytes
+    alg_flops = 8.0 * w["vlist_pairs"] * H
     had_tf = alg_flops / (had_ms * 1e-3) / 1e12
-    had_peak = PEAK_TFLOPS["fp32" if mixed else "fp64"]
+    peak = PEAK_TF["fp32" if mixed else "fp64"]
     kname = "k_m2l_tile" if mixed else "k_hadamard"
-    roofline = {"bound": "mfma", "kernel": kname, "achieved": round(had_tf, 3), "peak": had_peak,
-                "unit": "TFLOP/s", "frac": round(had_tf / had_peak, 4), "traffic": traffic_bytes(kname),
+    roofline = {"bound": "valu", "kernel": kname, "achieved": round(had_tf, 3), "peak": peak,
+                "unit": "TFLOP/s", "frac": round(had_tf / peak, 4), "traffic": traffic_bytes(kname),
                 "kernel_ms": round(had_ms, 4), "algorithmic_flops": alg_flops, "algorithmic_bytes": alg_bytes,
                 "intensity_flop_per_byte": round(alg_flops / alg_bytes, 2),
                 "hbm_gbs_at_algorithmic_bytes": round(alg_bytes / (had_ms * 1e-3) / 1e9, 1)}
-    # the near field (U/W/X S2T, M2T, L2T), the second-largest kernel: FP32/FP64
-    # vector bound, 11 flops per Laplace pair
     near_ms = phases["EvlUList"]
     near_tf = w["near_pairs"] * 11.0 / (near_ms * 1e-3) / 1e12
-    peak = PEAK_TFLOPS["fp32" if mixed else "fp64"]
     near = {"bound": "valu", "kernel": "k_near_lap_mx" if mixed else "k_near", "achieved": round(near_tf, 3),
             "peak": peak, "unit": "TFLOP/s", "frac": round(near_tf / peak, 4), "kernel_ms": round(near_ms, 4)}
 
@@ -200,12 +365,18 @@ def main():
     if not args.no_cpu:
         nue = np_ ** 3 - (np_ - 2) ** 3
         nuc = (np_ + 2) ** 3 - np_ ** 3
-        pu = f.operator(0).reshape(nuc, nue).T
-        pd = f.operator(1).reshape(nue, nue).T
+        pinv = (f.operator(0).reshape(nuc, nue).T, f.operator(1).reshape(nue, nue).T)
         sample = min(args.cpu_sample, n)
-        mps, secs = cpu_baseline(cfg, (pu, pd), sample, cfg_index)
+        mps, warm, cold = cpu_baseline(cfg, pinv, sample)
+        hi = host_info()
         cpu = {"value": round(mps, 5), "unit": "Mpoints/s", "cores": 1, "kind": "port",
-               "sample": "%d points of the same distribution, one evaluate, %.1f s" % (sample, secs)}
+               "sample": "%d points of the same distribution (np 6, ptsMax 128), warm second evaluate %.1f s "
+                         "(cold first %.1f s), 1 thread" % (sample, warm, cold),
+               "cpu_model": hi["cpu_model"], "nproc": hi["nproc"], "nproc_available": hi["nproc_available"]}
+    val = r["val"]
+    diff_fp64 = float(np.linalg.norm(val - r64["val"]) / np.linalg.norm(r64["val"])) if r64 else None
+    dt, dt64 = r["dt"], (r64["dt"] if r64 else None)
+    release(r)
 
     out = {
         "metric": METRIC,
@@ -220,19 +391,101 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": WORKLOAD_NAME[args.config] if not args.n else "%s (n=%d)" % (args.config, n),
+        "config": {"workload": WORKLOAD_NAME["c3"] if not args.n else "c3 (n=%d)" % n,
                    "kernel": kt, "np": np_, "ptsMax": pm, "maxLevel": ml, "precision": args.precision,
                    "n_points": n},
-        "rel_err": rel_err,
+        "rel_err": r["rel_err"],
+        "graph_ms_per_step": round(r["dt_graph"] * 1e3, 4),
         "fp64_ms_per_step": round(dt64 * 1e3, 4) if dt64 else None,
         "speedup_vs_fp64": round(dt64 / dt, 3) if dt64 else None,
         "rel_diff_vs_fp64": diff_fp64,
+        "setup_s": round(r["setup_s"], 3),
+        "direct_sum_ms": round(r["direct_ms"], 2),
         "phases_ms": {k: round(v, 4) for k, v in phases.items()},
         "roofline": roofline,
+        "phase_roofline": per_phase,
+        "eval_roofline_frac": round(total_roof / (sum(phases[k] for k in phases if k != "VListHadamard") * 1e-3), 4),
         "near_field": near,
         "cpu_baseline": cpu,
     }
+
+    if not args.no_extra and not args.n:
+        xs = min(args.steps, 5)
+        extra = {}
+        # the same workload at np 7: the smallest np that reaches 1e-6
+        r7_64 = R.run(cfg, n, "fp64", pos, nor, d_den, idx, steps=xs, warmup=1, np_override=7, tag="c3")
+        r7_64["f"].close()
+        r7 = R.run(cfg, n, "mixed", pos, nor, d_den, idx, steps=xs, warmup=1, np_override=7, tag="c3")
+        s = pair_summary(r7, r7_64, n)
+        s["meets_1e-6"] = bool(r7["rel_err"] <= 1e-6)
+        extra["c3_np7"] = s
+        release(r7, r7_64)
+        del pos, nor, den, d_den
+        gc.collect()
+        torch.cuda.empty_cache()
+        # the 32,768-point uniform case, FP64 (the CPU-runnable case)
+        c = CONFIGS["c1"]
+        p1, n1_, dn1, dd1, i1 = R.inputs(c, c[6])
+        a = R.run(c, c[6], "fp64", p1, n1_, dd1, i1, steps=xs, warmup=1, tag="c1")
+        extra["c1"] = {"workload": WORKLOAD_NAME["c1"], "fp64_ms": round(a["dt"] * 1e3, 4),
+                       "fp64_graph_ms": round(a["dt_graph"] * 1e3, 4),
+                       "mpoints_s": round(c[6] / a["dt"] / 1e6, 3), "rel_err": a["rel_err"]}
+        release(a)
+        # Stokes double layer on 32 spheres, FP64 and mixed
+        c = CONFIGS["c2"]
+        p2, n2_, dn2, dd2, i2 = R.inputs(c, c[6])
+        b64 = R.run(c, c[6], "fp64", p2, n2_, dd2, i2, steps=xs, warmup=1, tag="c2")
+        b64["f"].close()
+        b = R.run(c, c[6], "mixed", p2, n2_, dd2, i2, steps=xs, warmup=1, tag="c2")
+        s = pair_summary(b, b64, c[6])
+        s["workload"] = WORKLOAD_NAME["c2"]
+        s["phases_ms_mixed"] = {k: round(v, 4) for k, v in b["phases"].items()}
+        s["phases_ms_fp64"] = {k: round(v, 4) for k, v in b64["phases"].items()}
+        wb = b["f"].work_counts()
+        pr, tr = phase_roofline(321, 6, wb, b["phases"], True, c[6], c[6])
+        s["phase_roofline_mixed"] = pr
+        s["eval_roofline_frac_mixed"] = round(
+            tr / (sum(b["phases"][k] for k in b["phases"] if k != "VListHadamard") * 1e-3), 4)
+        extra["c2"] = s
+        release(b, b64)
+        del p2, n2_, dn2, dd2
+        # the gradient workload: forward, transposed and field passes
+        extra["c4"] = bench_gradient(R, xs)
+        out.update(extra)
     print(json.dumps(out), flush=True)
+
+
+def bench_gradient(R, steps):
+    import torch
+    import kifmm
+    from kifmm import workloads as W
+    c = CONFIGS["c4"]
+    kt, coefs, np_, pm, ml, gen, n, ci = c
+    pos, _ = make_points(gen, n, ci)
+    res = {"workload": WORKLOAD_NAME["c4"]}
+    for prec in ("fp64", "mixed"):
+        f = kifmm.FMM3d(kt, coefs, np_, pm, ml, precision=prec)
+        f.set_points(pos)
+        f.setup()
+        sig = torch.from_numpy(W.densities(n, 1, W.seed(ci, 1))).to(R.dev)
+        out = torch.empty(n, dtype=torch.float64, device=R.dev)
+        adj = torch.empty(n, dtype=torch.float64, device=R.dev)
+        grad = torch.empty(3 * n, dtype=torch.float64, device=R.dev)
+        times = {}
+        for name, fn in (("forward", lambda: f.evaluate_device(sig, out)),
+                         ("adjoint", lambda: f.evaluate_adjoint_device(out, adj)),
+                         ("field", lambda: f.field_device(sig, grad))):
+            fn()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                fn()
+            torch.cuda.synchronize()
+            times[name] = round((time.perf_counter() - t0) / steps * 1e3, 4)
+        res[prec + "_ms"] = times
+        print("[c4 %s] %s" % (prec, json.dumps(times)), file=sys.stderr, flush=True)
+        f.close()
+    return res
 
 
 if __name__ == "__main__":

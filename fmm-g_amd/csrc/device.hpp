@@ -68,9 +68,9 @@ struct GemmArgs {
   // columns [zoff[z], zoff[z+1]); zoff == nullptr means one matrix, all columns
   const int* zoff; int nz; int64_t a_zstride; int max_zcol;
   // split-K (mixed path only): blockIdx.z = matrix * ksplit + c sums k in
-  // [c*kchunk, (c+1)*kchunk).  With work (ksplit * ncol * M doubles) the partial
-  // products go there and a second kernel sums them in chunk order into C
-  // (deterministic); without it they are added to C atomically.
+  // [c*kchunk, (c+1)*kchunk).  The partial products go to work (ksplit * ncol *
+  // M doubles, required) and a second kernel sums them in chunk order into C
+  // (deterministic).
   int ksplit; int kchunk;
   double* work;
 };
@@ -146,8 +146,17 @@ hipError_t launch_gemm_fma(const GemmArgs& g, hipStream_t s);
 hipError_t launch_fft_fwd32(const VlistArgs& v, float2* dspec, hipStream_t s);
 hipError_t launch_fft_inv32(const VlistArgs& v, const float2* espec, hipStream_t s);
 
-hipError_t launch_direct(int kt, const KParams<double>& p, int64_t ns, const double* spos,
-                         const double* snor, const double* sden, int nt, const double* tpos,
-                         double* tval, hipStream_t s);
+// FP64 direct sum at nt targets: sources in leaf order (spos, snor), densities
+// in the caller's order read through sperm (leaf position -> caller index).
+// The sources are cut into nchunk chunks of `chunk` (a multiple of 256); the
+// partial sums need work_doubles = nchunk * nt * TD doubles of workspace.
+struct DirectPlan {
+  int nchunk; int64_t chunk; int64_t work_doubles;
+};
+// nchunk == 0 when the workspace capacity cannot hold one chunk's partial sums.
+DirectPlan plan_direct(int64_t ns, int nt, int td, int64_t cap_doubles);
+hipError_t launch_direct(int kt, const KParams<double>& p, int64_t ns, const double* spos, const double* snor,
+                         const int* sperm, const double* sden, int nt, const double* tpos, const DirectPlan& d,
+                         double* work, double* tval, hipStream_t s);
 
 }  // namespace kifmm

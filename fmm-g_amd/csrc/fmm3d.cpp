@@ -623,7 +623,8 @@ int FMM3d::reserve_splitk(size_t doubles) {
   return check(splitk_d_.alloc(doubles * sizeof(double)), "alloc split-K workspace");
 }
 
-// The workspace for a split launch, or null (atomic adds) if it does not fit.
+// The workspace for a split launch, or null if it does not fit (the launch is
+// then rejected; setup reserves enough for every step it plans).
 double* FMM3d::splitk_for(const GemmArgs& g) {
   if (g.ksplit <= 1) return nullptr;
   const size_t need = size_t(g.ksplit) * size_t(g.ncol) * size_t(g.M) * sizeof(double);
@@ -754,9 +755,13 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
       else { g.rscale = ones_d_.as<double>(); g.rsd = 1; }
       g.ncol = st.ncol; g.accumulate = st.accumulate;
       // mixed path: launches with very few workgroups split K into chunks of
-      // whole 16-deep slabs; the partial products are added atomically into
-      // outputs that start at zero (pinv, M2M) or accumulate (L2L)
-      if (mixed) { split_k(g); g.work = splitk_for(g); }
+      // whole 16-deep slabs; the chunks' partial products go to the workspace
+      // and are summed in chunk order into the output
+      if (mixed) {
+        split_k(g);
+        g.work = splitk_for(g);
+        if (g.ksplit > 1 && !g.work) return fail(-6, "split-K workspace smaller than the launch needs");
+      }
       int r = check(mixed ? launch_gemm_fma(g, s) : launch_gemm(g, s), "level GEMM");
       if (r) return r;
     }
@@ -840,14 +845,28 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
 int FMM3d::direct(const double* d_src_den, int nchk, const double* d_chk_pos, double* d_chk_val,
                   hipStream_t s) {
   if (!setup_done_) return fail(-3, "direct called before setup");
+  if (nchk < 0) return fail(-2, "negative check-target count");
+  if (nchk == 0) return 0;
+  if ((ns_ > 0 && !d_src_den) || !d_chk_pos || !d_chk_val) return fail(-2, "null density, position or value pointer");
+  const int TD = trgDOF();
+  // the workspace grows to what the plan asks for (at most 64 chunks' worth
+  // beyond one chunk of partial sums per target)
+  const int64_t cap = std::max<int64_t>(int64_t(nchk) * TD * 64, int64_t(1) << 20);
+  DirectPlan d = plan_direct(ns_, nchk, TD, cap);
+  if (d.nchunk <= 0) return fail(-6, "direct sum plan failed");
   int rc;
-  if (!direct_pos_d_.bytes()) {
-    if ((rc = check(direct_pos_d_.upload(spos_.data(), sizeof(double) * spos_.size()), "upload"))) return rc;
-    if ((rc = check(direct_nor_d_.upload(snor_.data(), sizeof(double) * snor_.size()), "upload"))) return rc;
-  }
+  if (size_t(d.work_doubles) * sizeof(double) > direct_work_d_.bytes())
+    if ((rc = check(direct_work_d_.alloc(size_t(d.work_doubles) * sizeof(double)), "alloc direct-sum workspace")))
+      return rc;
+  return direct_with(d_src_den, nchk, d_chk_pos, d_chk_val, d, direct_work_d_.as<double>(), s);
+}
+
+int FMM3d::direct_with(const double* d_src_den, int nchk, const double* d_chk_pos, double* d_chk_val,
+                       const DirectPlan& d, double* work, hipStream_t s) {
   const KParams<double> pu = make_params(kt_, coefs_.data());
-  return check(launch_direct(kt_, pu, ns_, direct_pos_d_.as<double>(), direct_nor_d_.as<double>(), d_src_den,
-                             nchk, d_chk_pos, d_chk_val, s), "direct sum");
+  return check(launch_direct(kt_, pu, ns_, spos_d_.as<double>(), snor_d_.as<double>(), sperm_d_.as<int>(), d_src_den,
+                             nchk, d_chk_pos, d, work, d_chk_val, s),
+               "direct sum");
 }
 
 }  // namespace kifmm

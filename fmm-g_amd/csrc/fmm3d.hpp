@@ -91,9 +91,14 @@ class FMM3d {
   // Gradient of the evaluated potential at the targets (3 values per target,
   // caller's order); Laplace single layer only.
   int field(const double* d_src_den, double* d_trg_grad, hipStream_t stream);
-  // FP64 direct sum at the given targets (device pointers, caller's order).
+  // FP64 direct sum at the given targets (device pointers, caller's order);
+  // the partial-sum workspace is the handle's, grown on demand.
   int direct(const double* d_src_den, int nchk, const double* d_chk_pos, double* d_chk_val,
              hipStream_t stream);
+  // Same with a caller-planned chunking and a caller-provided workspace of
+  // d.work_doubles doubles (allocates nothing).
+  int direct_with(const double* d_src_den, int nchk, const double* d_chk_pos, double* d_chk_val,
+                  const DirectPlan& d, double* work, hipStream_t stream);
 
   const Tree& tree() const { return tree_; }
   const Operators* ops() const { return ops_.get(); }
@@ -192,7 +197,7 @@ class FMM3d {
   DeviceBuffer nt_chunks_d_, nt_items_d_, ue_t_tasks_d_, de_t_tasks_d_, surf_t_items_d_, tbeg_d_, tnum_d_;
   DeviceBuffer m2mT_d_, pupT_d_, pdnT_d_, l2lT_d_;
   DeviceBuffer wup_d_;  // mixed path: P_up [M_0 ... M_7]
-  DeviceBuffer direct_pos_d_, direct_nor_d_;
+  DeviceBuffer direct_work_d_;  // direct sum: partial sums of the source chunks
   DeviceBuffer m2l32_d_, dspec32_d_, espec32_d_;  // mixed path: FP32 spectra
   std::vector<hipEvent_t> events_;
   bool use_graph_ = false;

@@ -274,9 +274,14 @@ int FMM3d::evaluateAdjoint(const double* d_trg_in, double* d_src_out, hipStream_
       if (stp.rs_off >= 0) { g.rscale = scales_d_.as<double>() + stp.rs_off; g.rsd = op.sd; }
       else { g.rscale = ones_d_.as<double>(); g.rsd = 1; }
       g.ncol = stp.ncol; g.accumulate = 1;
-      // every adjoint step accumulates, so K chunks may add atomically; the FP64
-      // mode keeps one workgroup per output tile and is deterministic
-      if (precision_ == PREC_MIXED) { split_k(g); g.work = splitk_for(g); }
+      // mixed mode: few-tile launches split K, the chunks' partial products are
+      // summed in chunk order (deterministic, as the FP64 mode's one workgroup
+      // per output tile)
+      if (precision_ == PREC_MIXED) {
+        split_k(g);
+        g.work = splitk_for(g);
+        if (g.ksplit > 1 && !g.work) return fail(-6, "split-K workspace smaller than the launch needs");
+      }
       int r = check(launch_gemm_fma(g, s), "adjoint level GEMM");
       if (r) return r;
     }

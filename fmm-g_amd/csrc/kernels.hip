@@ -10,6 +10,7 @@
 //   write back (:1490-1510)                               k_scatter
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "device.hpp"
@@ -672,31 +673,70 @@ hipError_t launch_fft_inv32(const VlistArgs& v, const float2* espec, hipStream_t
 // FP64 direct sum at sampled targets (dense3d_mpi.cpp:122-129): one workgroup
 // per target, lanes stride over the sources, LDS tree reduction.
 // ---------------------------------------------------------------------------
+// FP64 direct sum (Dense3d_MPI::evaluate, dense3d_mpi.cpp:75-141) at nt
+// check targets over all sources.  Workgroup (x, y) takes 256 targets (one per
+// thread) and source chunk y; the chunk's sources (leaf order, density read
+// through the source permutation) are staged through LDS 256 at a time and
+// every thread sweeps them with the exact kernel block (r < 1e-8 excluded).
+// Each chunk writes its partial sums to part[y][target][TD]; k_direct_sum adds
+// the chunks in chunk order (deterministic for a given chunk count).
 template <int KT>
-__global__ __launch_bounds__(256) void k_direct(KParams<double> p, int64_t ns, const double* __restrict__ spos,
-                                                const double* __restrict__ snor, const double* __restrict__ sden,
-                                                const double* __restrict__ tpos, double* __restrict__ tval) {
+__global__ __launch_bounds__(256) void k_direct_part(KParams<double> p, int64_t ns, int64_t chunk,
+                                                     const double* __restrict__ spos, const double* __restrict__ snor,
+                                                     const int* __restrict__ sperm, const double* __restrict__ sden,
+                                                     int nt, const double* __restrict__ tpos,
+                                                     double* __restrict__ part) {
   constexpr int SD = src_dof(KT), TD = trg_dof(KT);
-  __shared__ double red[256 * TD];
-  const int i = blockIdx.x;
-  const double tx = tpos[3 * i], ty = tpos[3 * i + 1], tz = tpos[3 * i + 2];
+  constexpr bool NOR = needs_normals_ct(KT);
+  __shared__ double sx[256], sy[256], sz[256], snx[NOR ? 256 : 1], sny[NOR ? 256 : 1], snz[NOR ? 256 : 1];
+  __shared__ double sdn[256 * SD];
+  const int tid = threadIdx.x;
+  const int i = blockIdx.x * 256 + tid;
+  const int ic = i < nt ? i : nt - 1;
+  const double tx = tpos[3 * int64_t(ic)], ty = tpos[3 * int64_t(ic) + 1], tz = tpos[3 * int64_t(ic) + 2];
+  const int64_t j0 = int64_t(blockIdx.y) * chunk, j1 = j0 + chunk < ns ? j0 + chunk : ns;
   double acc[TD];
+#pragma unroll
   for (int t = 0; t < TD; t++) acc[t] = 0.0;
-  for (int64_t j = threadIdx.x; j < ns; j += 256) {
-    double blk[SD * TD];
-    kblock<KT, double>(p, tx - spos[3 * j], ty - spos[3 * j + 1], tz - spos[3 * j + 2], snor[3 * j],
-                       snor[3 * j + 1], snor[3 * j + 2], blk);
-    for (int t = 0; t < TD; t++)
-      for (int s = 0; s < SD; s++) acc[t] += blk[t * SD + s] * sden[j * SD + s];
-  }
-  for (int t = 0; t < TD; t++) red[t * 256 + threadIdx.x] = acc[t];
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (int(threadIdx.x) < w)
-      for (int t = 0; t < TD; t++) red[t * 256 + threadIdx.x] += red[t * 256 + threadIdx.x + w];
+  for (int64_t b = j0; b < j1; b += 256) {
+    const int64_t j = b + tid;
     __syncthreads();
+    if (j < j1) {
+      sx[tid] = spos[3 * j]; sy[tid] = spos[3 * j + 1]; sz[tid] = spos[3 * j + 2];
+      if constexpr (NOR) { snx[tid] = snor[3 * j]; sny[tid] = snor[3 * j + 1]; snz[tid] = snor[3 * j + 2]; }
+      const int64_t o = sperm[j];
+#pragma unroll
+      for (int q = 0; q < SD; q++) sdn[tid * SD + q] = sden[o * SD + q];
+    } else {  // padding: far away, zero density
+      sx[tid] = sy[tid] = sz[tid] = 1e18;
+      if constexpr (NOR) { snx[tid] = sny[tid] = snz[tid] = 0.0; }
+#pragma unroll
+      for (int q = 0; q < SD; q++) sdn[tid * SD + q] = 0.0;
+    }
+    __syncthreads();
+    const int cnt = j1 - b < 256 ? int(j1 - b) : 256;
+    for (int k = 0; k < cnt; k++) {
+      double blk[SD * TD], nx = 0.0, ny = 0.0, nz = 0.0;
+      if constexpr (NOR) { nx = snx[k]; ny = sny[k]; nz = snz[k]; }
+      kblock<KT, double>(p, tx - sx[k], ty - sy[k], tz - sz[k], nx, ny, nz, blk);
+#pragma unroll
+      for (int t = 0; t < TD; t++)
+#pragma unroll
+        for (int q = 0; q < SD; q++) acc[t] += blk[t * SD + q] * sdn[k * SD + q];
+    }
   }
-  if (threadIdx.x < TD) tval[int64_t(i) * TD + threadIdx.x] = red[threadIdx.x * 256];
+  if (i < nt) {
+#pragma unroll
+    for (int t = 0; t < TD; t++) part[(int64_t(blockIdx.y) * nt + i) * TD + t] = acc[t];
+  }
+}
+
+__global__ void k_direct_sum(const double* __restrict__ part, int nchunk, int64_t n, double* __restrict__ out) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v = 0.0;
+  for (int c = 0; c < nchunk; c++) v += part[int64_t(c) * n + i];
+  out[i] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -726,15 +766,41 @@ hipError_t launch_surface(int kt, const SurfArgs& a, const KParams<double>& pu, 
   return hipGetLastError();
 }
 
+DirectPlan plan_direct(int64_t ns, int nt, int td, int64_t cap_doubles) {
+  DirectPlan d{};
+  if (nt <= 0) return d;
+  const int64_t per_chunk = int64_t(nt) * td;
+  if (per_chunk > cap_doubles) return d;  // nchunk 0: the workspace cannot hold one chunk
+  const int64_t tiles = (nt + 255) / 256;
+  const int64_t blocks = std::max<int64_t>(1, (ns + 255) / 256);
+  // about 8 workgroups per CU over 256 CUs, within the workspace
+  int64_t nchunk = std::min<int64_t>({blocks, std::max<int64_t>(1, (2048 + tiles - 1) / tiles),
+                                      cap_doubles / per_chunk, 65535});
+  nchunk = std::max<int64_t>(1, nchunk);
+  d.chunk = ((ns + nchunk - 1) / nchunk + 255) / 256 * 256;
+  if (d.chunk == 0) d.chunk = 256;
+  d.nchunk = int(std::max<int64_t>(1, (ns + d.chunk - 1) / d.chunk));
+  d.work_doubles = int64_t(d.nchunk) * per_chunk;
+  return d;
+}
+
 hipError_t launch_direct(int kt, const KParams<double>& p, int64_t ns, const double* spos, const double* snor,
-                         const double* sden, int nt, const double* tpos, double* tval, hipStream_t s) {
+                         const int* sperm, const double* sden, int nt, const double* tpos, const DirectPlan& d,
+                         double* work, double* tval, hipStream_t s) {
   if (nt <= 0) return hipSuccess;
+  if (d.nchunk <= 0 || !work || d.chunk <= 0 || int64_t(d.nchunk) * d.chunk < ns) return hipErrorInvalidValue;
+  const dim3 grid(unsigned((nt + 255) / 256), unsigned(d.nchunk));
   switch (kt) {
-#define KIFMM_CASE(K) case K: k_direct<K><<<dim3(unsigned(nt)), dim3(256), 0, s>>>(p, ns, spos, snor, sden, tpos, tval); break;
+#define KIFMM_CASE(K) \
+    case K: k_direct_part<K><<<grid, dim3(256), 0, s>>>(p, ns, d.chunk, spos, snor, sperm, sden, nt, tpos, work); break;
     KIFMM_USER_KERNELS(KIFMM_CASE)
 #undef KIFMM_CASE
     default: return hipErrorInvalidValue;
   }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int64_t n = int64_t(nt) * trg_dof(kt);
+  k_direct_sum<<<dim3(unsigned((n + 255) / 256)), dim3(256), 0, s>>>(work, d.nchunk, n, tval);
   return hipGetLastError();
 }
 

@@ -677,10 +677,10 @@ hipError_t launch_hadamard_mixed(const VlistArgs& v, const float2* dspec, const 
 // MFMAs per wave.  108 VGPRs and 33 KB of LDS: 4 waves per SIMD (the same tile
 // on 4 waves of ten row blocks each needed 168 VGPRs, 3 waves per SIMD, and was
 // 13% slower on the down pass).
-// With ksplit > 1, blockIdx.z = matrix * ksplit + K chunk and the partial
-// products are added to C with FP64 atomics, so a level with few columns still
-// spreads over enough workgroups (C must then hold its starting value: zero
-// for the pinv and M2M outputs, which are written once).  Operand
+// With ksplit > 1, blockIdx.z = matrix * ksplit + K chunk: each chunk writes
+// its partial product to g.work, and k_splitk_reduce sums the chunks in chunk
+// order into C (deterministic), so a level with few columns still spreads over
+// enough workgroups.  A split launch without a workspace is rejected.  Operand
 // layout of the 16x16x4 instruction: lane l supplies A[l%16][l/16] and
 // B[l/16][l%16], and its result register r holds D[4r + l/16][l%16] (rows
 // interleaved across the four lane groups).
@@ -775,8 +775,7 @@ __global__ __launch_bounds__(GM_T, 2) void k_gemm_mfma(GemmArgs g) {
       const int m = m0 + 80 * wr + 16 * mi + 4 * r + lk;
       if (m >= g.M) continue;
       const double v = g.rscale[m % g.rsd] * acc[mi][r];
-      if (split && g.work) g.work[(int64_t(zk) * g.ncol + n) * g.M + m] = v;
-      else if (split) unsafeAtomicAdd(col + m, v);
+      if (split) g.work[(int64_t(zk) * g.ncol + n) * g.M + m] = v;
       else col[m] = g.accumulate ? col[m] + v : v;
     }
 }
@@ -796,11 +795,11 @@ __global__ void k_splitk_reduce(const double* __restrict__ work, int ks, int nco
 static hipError_t launch_gemm_mfma(const GemmArgs& g, hipStream_t s) {
   if (g.ncol <= 0) return hipSuccess;
   const bool split = g.ksplit > 1;
-  if (split && (g.kchunk <= 0 || int64_t(g.ksplit) * g.kchunk < g.K)) return hipErrorInvalidValue;
+  if (split && (g.kchunk <= 0 || int64_t(g.ksplit) * g.kchunk < g.K || !g.work)) return hipErrorInvalidValue;
   const int nz = (g.zoff ? g.nz : 1) * (split ? g.ksplit : 1);
   dim3 grid(unsigned((g.M + GM_M - 1) / GM_M), unsigned((g.max_zcol + GM_N - 1) / GM_N), unsigned(nz));
   k_gemm_mfma<<<grid, dim3(GM_T), 0, s>>>(g);
-  if (split && g.work) {
+  if (split) {
     const int64_t n = int64_t(g.ncol) * g.M;
     k_splitk_reduce<<<dim3(unsigned((n + 255) / 256)), dim3(256), 0, s>>>(g.work, g.ksplit, g.ncol, g.M, g.cidx, g.C,
                                                                            g.ldc, g.accumulate);

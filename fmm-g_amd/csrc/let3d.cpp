@@ -180,7 +180,7 @@ int build_tree(const double* spos, int64_t ns, const double* tpos, int64_t nt, c
       double v = spos[3 * i + d];
       if (!(v >= lo[d] && v <= lo[d] + 2 * t.rad0)) {
         if (err) *err = "source point " + std::to_string(i) + " lies outside the root box";
-        return -3;
+        return -2;
       }
     }
   for (int64_t i = 0; i < nt; i++)
@@ -188,7 +188,7 @@ int build_tree(const double* spos, int64_t ns, const double* tpos, int64_t nt, c
       double v = tpos[3 * i + d];
       if (!(v >= lo[d] && v <= lo[d] + 2 * t.rad0)) {
         if (err) *err = "target point " + std::to_string(i) + " lies outside the root box";
-        return -3;
+        return -2;
       }
     }
 

@@ -101,6 +101,27 @@ def _pd(a):
     return a.ctypes.data_as(P_D) if a is not None else None
 
 
+def _dev(t, n, what):
+    """Device pointer of a float64, contiguous GPU tensor holding exactly n
+    values; anything else raises before the library sees the pointer."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("%s must be a torch tensor" % what)
+    if not t.is_cuda:
+        raise TypeError("%s must be a GPU tensor" % what)
+    if t.dtype != torch.float64:
+        raise TypeError("%s must be float64, got %s" % (what, t.dtype))
+    if not t.is_contiguous():
+        raise ValueError("%s must be contiguous" % what)
+    if t.numel() != n:
+        raise ValueError("%s must hold %d values, got %d" % (what, n, t.numel()))
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    return C.c_void_p(stream) if stream else None
+
+
 def options_lookup(path, name):
     """Value of option `name` in a PETSc-style options file ('' for a flag, None if absent)."""
     buf = C.create_string_buffer(1024)
@@ -185,9 +206,9 @@ class FMM3d:
 
     def evaluate_device(self, d_src_den, d_trg_val, stream=None):
         """Device-resident float64 tensors (e.g. torch CUDA/HIP tensors)."""
-        self._chk(lib().kifmm_evaluate_device(self._h, C.c_void_p(d_src_den.data_ptr()),
-                                              C.c_void_p(d_trg_val.data_ptr()),
-                                              C.c_void_p(stream) if stream else None))
+        self._chk(lib().kifmm_evaluate_device(self._h, _dev(d_src_den, self.n_src * self.src_dof, "densities"),
+                                              _dev(d_trg_val, self.n_trg * self.trg_dof, "values"),
+                                              _stream(stream)))
 
     def evaluate_multi(self, src_den):
         """nrhs density vectors (rows of src_den) -> nrhs value vectors."""
@@ -205,9 +226,9 @@ class FMM3d:
 
     def evaluate_adjoint_device(self, d_trg_in, d_src_out, stream=None):
         """Transposed pass on device-resident float64 tensors."""
-        self._chk(lib().kifmm_evaluate_adjoint_device(self._h, C.c_void_p(d_trg_in.data_ptr()),
-                                                      C.c_void_p(d_src_out.data_ptr()),
-                                                      C.c_void_p(stream) if stream else None))
+        self._chk(lib().kifmm_evaluate_adjoint_device(self._h, _dev(d_trg_in, self.n_trg * self.trg_dof, "input"),
+                                                      _dev(d_src_out, self.n_src * self.src_dof, "output"),
+                                                      _stream(stream)))
 
     def field(self, src_den):
         """Gradient of the potential at the targets, shape (n_trg, 3)."""
@@ -218,16 +239,19 @@ class FMM3d:
 
     def field_device(self, d_src_den, d_trg_grad, stream=None):
         """Gradient pass on device-resident float64 tensors (3 per target)."""
-        self._chk(lib().kifmm_field_device(self._h, C.c_void_p(d_src_den.data_ptr()),
-                                           C.c_void_p(d_trg_grad.data_ptr()),
-                                           C.c_void_p(stream) if stream else None))
+        self._chk(lib().kifmm_field_device(self._h, _dev(d_src_den, self.n_src * self.src_dof, "densities"),
+                                           _dev(d_trg_grad, self.n_trg * 3, "gradients"),
+                                           _stream(stream)))
 
     def direct_device(self, d_src_den, d_chk_pos, d_chk_val, stream=None):
+        """FP64 direct sum at the check positions (device tensors)."""
+        if d_chk_pos.numel() % 3:
+            raise ValueError("check positions must hold 3 values per point")
         n = d_chk_pos.numel() // 3
-        self._chk(lib().kifmm_direct_device(self._h, C.c_void_p(d_src_den.data_ptr()), n,
-                                            C.c_void_p(d_chk_pos.data_ptr()),
-                                            C.c_void_p(d_chk_val.data_ptr()),
-                                            C.c_void_p(stream) if stream else None))
+        self._chk(lib().kifmm_direct_device(self._h, _dev(d_src_den, self.n_src * self.src_dof, "densities"), n,
+                                            _dev(d_chk_pos, 3 * n, "check positions"),
+                                            _dev(d_chk_val, n * self.trg_dof, "check values"),
+                                            _stream(stream)))
 
     def check(self, src_den, trg_val, numchk, seed=0):
         d = _f64(src_den)

@@ -23,6 +23,8 @@ class _FMMApply(torch.autograd.Function):
     def forward(ctx, den, f):
         if not den.is_cuda or den.dtype != torch.float64:
             raise TypeError("densities must be a float64 GPU tensor")
+        if den.numel() != f.n_src * f.src_dof:
+            raise ValueError("densities must hold %d values, got %d" % (f.n_src * f.src_dof, den.numel()))
         den = den.contiguous()
         out = torch.empty(f.n_trg * f.trg_dof, dtype=torch.float64, device=den.device)
         f.evaluate_device(den, out, stream=_stream())
@@ -32,7 +34,7 @@ class _FMMApply(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_out):
         f = ctx.f
-        g = grad_out.contiguous()
+        g = grad_out.to(torch.float64).contiguous()
         out = torch.empty(f.n_src * f.src_dof, dtype=torch.float64, device=g.device)
         f.evaluate_adjoint_device(g, out, stream=_stream())
         return out, None

@@ -59,6 +59,13 @@ def test_points_outside_root_box_rejected():
     with pytest.raises(kifmm.KifmmError) as e:
         f.setup_host()
     assert "outside the root box" in str(e.value)
+    assert e.value.code == -2  # KIFMM_ERR_ARG: bad input, not a call out of order
+    tp = W.uniform(30, 3)
+    tp[4] = [-0.25, 0.5, 0.5]
+    f.set_points(W.uniform(50, 2), None, tp)
+    with pytest.raises(kifmm.KifmmError) as e:
+        f.setup_host()
+    assert e.value.code == -2 and "target point 4" in str(e.value)
 
 
 def test_double_layer_needs_normals():

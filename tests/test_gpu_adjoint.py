@@ -73,10 +73,13 @@ def test_adjoint_duality_mixed(kt, coefs):
     assert dual_gap(f, 4) < 1e-5
 
 
-def test_adjoint_dense_vs_oracle():
-    """A formed from the oracle's evaluate, one unit density at a time."""
-    n, np_, pm = 240, 4, 6
-    print("case adjoint dense n=%d" % n, flush=True)
+@pytest.mark.parametrize("np_,bar", [(4, 1e-12), (6, 1e-10)])
+def test_adjoint_dense_vs_oracle(np_, bar):
+    """A formed from the oracle's evaluate, one unit density at a time.  At np 6
+    the bar is the duality bar: the pseudo-inverses (condition ~1e7) amplify the
+    round-off of the transposed summation order."""
+    n, pm = 240, 6
+    print("case adjoint dense n=%d np=%d" % (n, np_), flush=True)
     pos = W.gaussians(n, 44)
     nor = np.zeros_like(pos)
     f = kifmm.FMM3d(111, [1.0, 0.25], np_, pm, 30)
@@ -99,7 +102,7 @@ def test_adjoint_dense_vs_oracle():
     ref = A.T @ w
     r = np.linalg.norm(at - ref) / np.linalg.norm(ref)
     print("relative difference %.3e" % r, flush=True)
-    assert r < 1e-12
+    assert r < bar
 
 
 def direct_grad(pos, den, tgt):
@@ -107,7 +110,7 @@ def direct_grad(pos, den, tgt):
     closer than 1e-8 excluded, as the reference kernel does)."""
     d = tgt[:, None, :] - pos[None, :, :]
     r2 = np.einsum("ijk,ijk->ij", d, d)
-    inv3 = np.where(r2 < 1e-16, 0.0, r2 ** -1.5)
+    inv3 = np.where(r2 < 1e-16, 0.0, np.maximum(r2, 1e-16) ** -1.5)
     return -np.einsum("ij,j,ijk->ik", inv3, den, d) / (4 * np.pi)
 
 

@@ -311,3 +311,126 @@ def test_graph_replay(gpu, precision, bar):
         r = np.linalg.norm(got - ref) / np.linalg.norm(ref)
         print("replay %d relative difference %.3e" % (rep, r), flush=True)
         assert r <= bar
+
+
+def test_stokes_double_layer_spheres_np6():
+    """The sphere-surface workload, reduced: Stokes double layer (kernel 321,
+    equivalent kernel 301, sd 4) at np 6 on 32 non-overlapping spheres, ptsMax 64,
+    3-vector N(0,1) densities.  FP64 against the oracle at the parity bar; the
+    mixed path against the same oracle values (bar 3e-5: the FP32 near field of a
+    double layer on a surface, where r.n cancels for close pairs) and no less
+    accurate than FP64 against the direct sum beyond that difference."""
+    n = 30000
+    pos, nor = W.spheres(n, W.seed(1, 0), nsph=32)
+    den = W.densities(n, 3, W.seed(1, 1), kind="normal")
+    f, val, ref = run_pair(321, [1.0, 0.25], 6, 64, pos, nor, den)
+    assert rel(val, ref) <= PARITY
+    g = kifmm.FMM3d(321, [1.0, 0.25], 6, 64, 30, precision="mixed")
+    g.set_points(pos, nor)
+    g.setup()
+    mv = g.evaluate(den)
+    d = rel(mv, ref)
+    e64 = f.check(den, val, 2000, seed=W.seed(1, 3))
+    emx = g.check(den, mv, 2000, seed=W.seed(1, 3))
+    print("stokes DL np6: mixed vs oracle %.3e, errors vs direct fp64 %.3e mixed %.3e" % (d, e64, emx), flush=True)
+    assert d < 3e-5
+    assert emx <= e64 + 2 * d
+
+
+def test_mixed_vs_oracle_clustered_np6():
+    """The clustered Laplace workload's settings (16 Gaussians + 10% uniform,
+    np 6, ptsMax 128, maxLevel 14) at 40,000 points: the mixed path (FP32 near
+    field, S2M/S2L and V list; FP64 pseudo-inverses and sums) against the
+    oracle's FP64 restatement of the reference CPU evaluate on the same tree and
+    tables.  Bar 1e-6: FP32 arithmetic on the far field; a wrong list, scale or
+    table entry moves the result by 1e-3 or more."""
+    n = 40000
+    pos = W.gaussians(n, W.seed(2, 0))
+    den = W.densities(n, 1, W.seed(2, 1))
+    f = kifmm.FMM3d(111, [1.0, 0.25], 6, 128, 14, precision="mixed")
+    f.set_points(pos)
+    f.setup()
+    val = f.evaluate(den)
+    pu = f.operator(0).reshape(296, 152).T
+    pd = f.operator(1).reshape(152, 152).T
+    ops = O.Ops(111, [1.0, 0.25], 6, pu, pd)
+    tree = O.Tree(pos, None, pts_max=128, max_level=14)
+    ref = O.evaluate(tree, ops, 111, [1.0, 0.25], pos, np.zeros_like(pos), pos, den)
+    assert rel(val, ref) < 1e-6
+
+
+@pytest.mark.parametrize("kt,coefs", [(111, [1.0, 0.25]), (121, [1.0, 0.25]), (311, [1.0, 0.25]),
+                                      (321, [1.0, 0.25]), (322, [1.0, 0.25]), (511, [1.0, 0.3])])
+def test_direct_vs_oracle(gpu, kt, coefs):
+    """kifmm_direct_device (the FP64 direct sum every accuracy figure is measured
+    against, Dense3d_MPI::evaluate, dense3d_mpi.cpp:122-129) against the
+    oracle's direct sum at 1,500 targets, sources on sphere surfaces with
+    normals; coincident targets exercise the r < 1e-8 exclusion."""
+    import torch
+    n, m = 12000, 1500
+    pos, nor = W.spheres(n, 400 + kt, nsph=8)
+    rng = np.random.default_rng(401 + kt)
+    tgt = np.concatenate([pos[rng.choice(n, m // 2, replace=False)], W.uniform(m - m // 2, 402 + kt)])
+    den = W.densities(n, O.src_dof(kt), 403 + kt, kind="normal")
+    print("case direct kt=%d n=%d targets=%d" % (kt, n, m), flush=True)
+    f = kifmm.FMM3d(kt, coefs, 4, 64, 30)
+    f.set_points(pos, nor)
+    f.setup()
+    d_den = torch.from_numpy(den).to(gpu)
+    d_chk = torch.from_numpy(np.ascontiguousarray(tgt)).to(gpu)
+    d_out = torch.empty(m * f.trg_dof, dtype=torch.float64, device=gpu)
+    f.direct_device(d_den, d_chk, d_out)
+    torch.cuda.synchronize()
+    ref = O.direct(kt, coefs, pos, nor, den, tgt)
+    assert rel(d_out.cpu().numpy(), ref) <= 1e-13
+
+
+def test_check_vs_oracle_all_targets():
+    """kifmm_check over all targets (numchk >= N, fmm3d_check_mpi.cpp:46-52) is
+    the reference's relative 2-norm error (:159-165) against the oracle's
+    direct sum, computed here from the same values."""
+    n = 6000
+    pos = W.uniform(n, 410)
+    den = W.densities(n, 1, 411)
+    f = kifmm.FMM3d(111, [1.0, 0.25], 4, 32, 30)
+    f.set_points(pos)
+    f.setup()
+    val = f.evaluate(den)
+    err = f.check(den, val, n + 5, seed=1)
+    ref = O.direct(111, [1.0, 0.25], pos, np.zeros_like(pos), den, pos)
+    want = np.linalg.norm(val - ref) / np.linalg.norm(ref)
+    print("check %.6e oracle %.6e" % (err, want), flush=True)
+    assert abs(err - want) <= 1e-9 * want
+
+
+def test_handle_reuse_new_points(gpu):
+    """set_points + setup again on one handle with a larger point set: the
+    evaluate, the direct sum and the check use the new points (against a fresh
+    handle on the same points, bitwise)."""
+    import torch
+    f = kifmm.FMM3d(111, [1.0, 0.25], 4, 32, 30)
+    a = W.uniform(3000, 420)
+    f.set_points(a)
+    f.setup()
+    f.check(W.densities(3000, 1, 421), f.evaluate(W.densities(3000, 1, 421)), 200, seed=3)
+    b = W.gaussians(9000, 422)
+    den = W.densities(9000, 1, 423)
+    f.set_points(b)
+    f.setup()
+    g = kifmm.FMM3d(111, [1.0, 0.25], 4, 32, 30)
+    g.set_points(b)
+    g.setup()
+    vf, vg = f.evaluate(den), g.evaluate(den)
+    assert np.array_equal(vf, vg)
+    assert f.check(den, vf, 500, seed=4) == g.check(den, vg, 500, seed=4)
+    d_den = torch.from_numpy(den).to(gpu)
+    d_chk = torch.from_numpy(np.ascontiguousarray(b[:700])).to(gpu)
+    outs = []
+    for h in (f, g):
+        o = torch.empty(700, dtype=torch.float64, device=gpu)
+        h.direct_device(d_den, d_chk, o)
+        outs.append(o)
+    torch.cuda.synchronize()
+    assert torch.equal(outs[0], outs[1])
+    ref = O.direct(111, [1.0, 0.25], b, np.zeros_like(b), den, b[:700])
+    assert rel(outs[0].cpu().numpy(), ref) <= 1e-13

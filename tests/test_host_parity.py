@@ -47,6 +47,23 @@ def test_tree_and_lists_bit_exact(case):
         assert np.array_equal(tr["lists"][name][1], ot.lists[name][1]), name
 
 
+def test_tree_and_lists_bit_exact_clustered_8m():
+    """The whole point set of the 8,388,608-point clustered Laplace workload
+    (16 Gaussians + 10% uniform, ptsMax 128, maxLevel 14): octree, tags,
+    packing and all four lists equal the oracle's bit for bit (about 20 s)."""
+    pos = W.gaussians(8388608, W.seed(2, 0))
+    f, tr = product_tree(pos, None, np_=6, pts_max=128, max_level=14)
+    ot = O.Tree(pos, None, pts_max=128, max_level=14)
+    assert tr["parent"].size == ot.n_nodes
+    for k in ["parent", "child", "depth", "tag", "src_beg", "src_num", "trg_beg", "trg_num",
+              "src_perm", "trg_perm"]:
+        assert np.array_equal(tr[k], getattr(ot, k)), k
+    assert np.array_equal(tr["path"], ot.path)
+    for name in "uvwx":
+        assert np.array_equal(tr["lists"][name][0], ot.lists[name][0]), name
+        assert np.array_equal(tr["lists"][name][1], ot.lists[name][1]), name
+
+
 @pytest.mark.parametrize("kt,np_,coefs", [(111, 4, [1, .25]), (111, 6, [1, .25]), (321, 4, [1, .25]),
                                           (321, 6, [1, .25]), (511, 4, [1, .3]), (511, 8, [1, .3])])
 def test_operator_tables(kt, np_, coefs):
