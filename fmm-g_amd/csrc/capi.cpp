@@ -103,6 +103,19 @@ int kifmm_set_points(kifmm_handle h, int64_t n_src, const double* src_pos, const
   return h->fmm.setPoints(n_src, src_pos, src_nor, n_trg, trg_pos);
 }
 
+int kifmm_set_points_batch(kifmm_handle h, int nprob, const int64_t* n_src, const double* src_pos,
+                           const double* src_nor, const int64_t* n_trg, const double* trg_pos) {
+  if (!h) return set_err(nullptr, KIFMM_ERR_ARG, "null handle");
+  if (nprob < 1 || !n_src) return set_err(h, KIFMM_ERR_ARG, "a batch needs at least one problem and its point counts");
+  int64_t S = 0;
+  for (int p = 0; p < nprob; p++) S += n_src[p] > 0 ? n_src[p] : 0;
+  if (kifmm::needs_normals(h->fmm.kernelType()) && S > 0 && !src_nor)
+    return set_err(h, KIFMM_ERR_ARG, "kernel needs source normals");
+  if (!trg_pos) return h->fmm.setPointsBatch(nprob, n_src, src_pos, src_nor, n_src, src_pos);
+  if (!n_trg) return set_err(h, KIFMM_ERR_ARG, "target counts missing");
+  return h->fmm.setPointsBatch(nprob, n_src, src_pos, src_nor, n_trg, trg_pos);
+}
+
 int kifmm_setup(kifmm_handle h) {
   if (!h) return set_err(nullptr, KIFMM_ERR_ARG, "null handle");
   return h->fmm.setup();

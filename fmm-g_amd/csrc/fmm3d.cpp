@@ -69,8 +69,29 @@ int FMM3d::setPoints(int64_t ns, const double* spos, const double* snor, int64_t
   if (snor) snor_.assign(snor, snor + 3 * ns);
   else snor_.assign(3 * ns, 0.0);
   tpos_.assign(tpos, tpos + 3 * nt);
+  prob_ns_.clear();
+  prob_nt_.clear();
   setup_done_ = false;
   host_ready_ = false;
+  return 0;
+}
+
+int FMM3d::setPointsBatch(int nprob, const int64_t* ns, const double* spos, const double* snor, const int64_t* nt,
+                          const double* tpos) {
+  if (nprob < 1 || !ns || !nt) return fail(-2, "a batch needs at least one problem and its point counts");
+  int64_t S = 0, T = 0;
+  for (int p = 0; p < nprob; p++) {
+    if (ns[p] < 0 || nt[p] < 0) return fail(-2, "negative point count in problem " + std::to_string(p));
+    S += ns[p];
+    T += nt[p];
+  }
+  if (S > INT32_MAX || T > INT32_MAX) return fail(-2, "too many points for 32-bit indices");
+  int rc = setPoints(S, spos, snor, T, tpos);
+  if (rc) return rc;
+  if (nprob > 1) {
+    prob_ns_.assign(ns, ns + nprob);
+    prob_nt_.assign(nt, nt + nprob);
+  }
   return 0;
 }
 
@@ -97,8 +118,27 @@ int FMM3d::setupHost() {
   if (coefs_.empty()) return fail(-2, "kernel coefficients missing");
   if (np_ < 3 || np_ > 8) return fail(-2, "np must lie in [3,8]");
   if (precision_ != PREC_FP64 && precision_ != PREC_MIXED) return fail(-2, "unknown precision mode");
-  int rc = build_tree(spos_.data(), ns_, tpos_.data(), nt_, ctr_, root_level_, pts_max_, max_level_, tree_, &err_);
-  if (rc) return rc;
+  int rc = 0;
+  if (prob_ns_.empty()) {
+    rc = build_tree(spos_.data(), ns_, tpos_.data(), nt_, ctr_, root_level_, pts_max_, max_level_, tree_, &err_);
+    if (rc) return rc;
+  } else {
+    // one tree per problem (problems in parallel, each builder serial inside),
+    // then one numbering for the forest
+    const int P = int(prob_ns_.size());
+    std::vector<int64_t> so(P, 0), to(P, 0);
+    for (int p = 1; p < P; p++) { so[p] = so[p - 1] + prob_ns_[p - 1]; to[p] = to[p - 1] + prob_nt_[p - 1]; }
+    std::vector<Tree> parts(P);
+    std::vector<int> prc(P, 0);
+    std::vector<std::string> perr(P);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int p = 0; p < P; p++)
+      prc[p] = build_tree(spos_.data() + 3 * so[p], prob_ns_[p], tpos_.data() + 3 * to[p], prob_nt_[p], ctr_,
+                          root_level_, pts_max_, max_level_, parts[p], &perr[p]);
+    for (int p = 0; p < P; p++)
+      if (prc[p]) return fail(prc[p], "problem " + std::to_string(p) + ": " + perr[p]);
+    merge_trees(parts, so, to, tree_);
+  }
   std::string e;
   std::vector<double> mmc = coefs_;
   if (kt_ == 312 || kt_ == 322) mmc.clear();  // fmm3d_setup_mpi.cpp:119,121

@@ -77,6 +77,13 @@ class FMM3d {
   // Points are copied (the caller keeps ownership); normals may be null for
   // kernels that do not use them.
   int setPoints(int64_t ns, const double* spos, const double* snor, int64_t nt, const double* tpos);
+  // Many independent problems in one handle (a forest): problem p has ns[p]
+  // sources and nt[p] targets; the arrays hold the problems one after another.
+  // setup() builds one tree per problem and numbers them as one forest, so
+  // every pass of evaluate() runs one launch over all problems' boxes.
+  int setPointsBatch(int nprob, const int64_t* ns, const double* spos, const double* snor, const int64_t* nt,
+                     const double* tpos);
+  int numProblems() const { return prob_ns_.empty() ? 1 : int(prob_ns_.size()); }
 
   // Host half of setup(): octree, lists, operator tables and task tables, no
   // device calls (usable on a machine without a GPU).
@@ -141,6 +148,7 @@ class FMM3d {
   double ctr_[3] = {0.5, 0.5, 0.5};
   int64_t ns_ = 0, nt_ = 0;
   std::vector<double> spos_, snor_, tpos_;
+  std::vector<int64_t> prob_ns_, prob_nt_;  // forest: points per problem (empty: one problem)
   bool setup_done_ = false, host_ready_ = false, timing_ = false;
   std::string err_;
   Tree tree_;

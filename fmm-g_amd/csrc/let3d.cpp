@@ -285,4 +285,54 @@ int build_tree(const double* spos, int64_t ns, const double* tpos, int64_t nt, c
   return 0;
 }
 
+void merge_trees(std::vector<Tree>& parts, const std::vector<int64_t>& src_off, const std::vector<int64_t>& trg_off,
+                 Tree& out) {
+  out = Tree();
+  if (parts.empty()) return;
+  out.max_level = parts[0].max_level;
+  out.root_level = parts[0].root_level;
+  out.rad0 = parts[0].rad0;
+  for (int d = 0; d < 3; d++) out.ctr[d] = parts[0].ctr[d];
+  int64_t nn = 0, nl[4] = {0, 0, 0, 0};
+  for (const Tree& t : parts) {
+    nn += t.n_nodes;
+    for (int k = 0; k < 4; k++) nl[k] += int64_t(t.lst[k].size());
+  }
+  out.n_nodes = int(nn);
+  for (int k = 0; k < 4; k++) {
+    out.off[k].reserve(size_t(nn) + 1);
+    out.off[k].push_back(0);
+    out.lst[k].reserve(size_t(nl[k]));
+  }
+  int node0 = 0;
+  for (size_t p = 0; p < parts.size(); p++) {
+    Tree& t = parts[p];
+    const int so = int(src_off[p]), to = int(trg_off[p]);
+    auto shift = [](std::vector<int>& dst, const std::vector<int>& src, int by) {
+      for (int v : src) dst.push_back(v >= 0 ? v + by : v);
+    };
+    shift(out.parent, t.parent, node0);
+    shift(out.child, t.child, node0);
+    out.depth.insert(out.depth.end(), t.depth.begin(), t.depth.end());
+    out.path.insert(out.path.end(), t.path.begin(), t.path.end());
+    out.tag.insert(out.tag.end(), t.tag.begin(), t.tag.end());
+    out.src_cnt.insert(out.src_cnt.end(), t.src_cnt.begin(), t.src_cnt.end());
+    out.trg_cnt.insert(out.trg_cnt.end(), t.trg_cnt.begin(), t.trg_cnt.end());
+    shift(out.src_beg, t.src_beg, so);
+    shift(out.trg_beg, t.trg_beg, to);
+    out.src_num.insert(out.src_num.end(), t.src_num.begin(), t.src_num.end());
+    out.trg_num.insert(out.trg_num.end(), t.trg_num.begin(), t.trg_num.end());
+    shift(out.src_perm, t.src_perm, so);
+    shift(out.trg_perm, t.trg_perm, to);
+    for (int k = 0; k < 4; k++) {
+      const int64_t base = out.off[k].back();
+      for (int i = 0; i < t.n_nodes; i++) out.off[k].push_back(base + t.off[k][size_t(i) + 1]);
+      shift(out.lst[k], t.lst[k], node0);
+    }
+    out.max_depth = std::max(out.max_depth, t.max_depth);
+    node0 += t.n_nodes;
+    t = Tree();  // release the part
+  }
+}
+
 }  // namespace kifmm

@@ -42,4 +42,13 @@ struct Tree {
 int build_tree(const double* spos, int64_t ns, const double* tpos, int64_t nt, const double ctr[3],
                int root_level, int pts_max, int max_level, Tree& t, std::string* err);
 
+// Forest of independent problems: the trees of parts[0..P) in one numbering.
+// Problem p's nodes follow those of problems 0..p-1 (node offset = their node
+// count), and its sources and targets follow theirs in the packed and in the
+// caller's order (src_off[p], trg_off[p]: the points of problems before p).
+// Lists, tags and ranges keep their meaning inside each tree; no list crosses
+// a tree.  The parts share ctr, root level and maxLevel.  parts is consumed.
+void merge_trees(std::vector<Tree>& parts, const std::vector<int64_t>& src_off, const std::vector<int64_t>& trg_off,
+                 Tree& out);
+
 }  // namespace kifmm

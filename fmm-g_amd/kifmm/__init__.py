@@ -34,6 +34,7 @@ SIGNATURES = {
     "kifmm_create_from_options": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(H)]),
     "kifmm_destroy": (C.c_int, [H]),
     "kifmm_set_points": (C.c_int, [H, C.c_int64, P_D, P_D, C.c_int64, P_D]),
+    "kifmm_set_points_batch": (C.c_int, [H, C.c_int, P_L, P_D, P_D, P_L, P_D]),
     "kifmm_setup": (C.c_int, [H]),
     "kifmm_setup_host": (C.c_int, [H]),
     "kifmm_src_dof": (C.c_int, [H]),
@@ -190,6 +191,26 @@ class FMM3d:
         sn = None if src_nor is None else _f64(src_nor, sp.size)
         self.n_src, self.n_trg = sp.size // 3, tp.size // 3
         self._chk(lib().kifmm_set_points(self._h, self.n_src, _pd(sp), _pd(sn), self.n_trg, _pd(tp)))
+
+    def set_points_batch(self, src_pos, src_nor=None, trg_pos=None):
+        """Many independent problems in one handle: src_pos (and src_nor,
+        trg_pos) are lists with one (n_p, 3) array per problem; trg_pos None
+        means targets = sources.  Densities and values are then the problems'
+        vectors one after another."""
+        ns = np.array([len(p) for p in src_pos], np.int64)
+        sp = _f64(np.concatenate([np.asarray(p, np.float64).reshape(-1, 3) for p in src_pos])).reshape(-1)
+        sn = None
+        if src_nor is not None:
+            sn = _f64(np.concatenate([np.asarray(p, np.float64).reshape(-1, 3) for p in src_nor]), sp.size)
+        if trg_pos is None:
+            nt, tp = ns, None
+        else:
+            nt = np.array([len(p) for p in trg_pos], np.int64)
+            tp = _f64(np.concatenate([np.asarray(p, np.float64).reshape(-1, 3) for p in trg_pos])).reshape(-1)
+        self.n_src, self.n_trg = int(ns.sum()), int(nt.sum())
+        self.problem_src, self.problem_trg = ns, nt
+        self._chk(lib().kifmm_set_points_batch(self._h, len(ns), ns.ctypes.data_as(P_L), _pd(sp), _pd(sn),
+                                               nt.ctypes.data_as(P_L), _pd(tp)))
 
     def setup(self):
         self._chk(lib().kifmm_setup(self._h))

@@ -55,6 +55,18 @@ int kifmm_destroy(kifmm_handle h);
 int kifmm_set_points(kifmm_handle h, int64_t n_src, const double *src_pos, const double *src_nor,
                      int64_t n_trg, const double *trg_pos);
 
+/* Many independent problems in one handle, the batched form of the point
+ * setters above (the reference builds one FMM3d_MPI per problem, tt0.cpp:429-518;
+ * no batched counterpart): problem p has n_src[p] sources and n_trg[p]
+ * targets, and the arrays hold the problems one after another (positions and
+ * normals 3 per point).  trg_pos NULL (n_trg ignored) means targets = sources.
+ * kifmm_setup builds one octree per problem and numbers them as one forest;
+ * each pass of kifmm_evaluate* then runs one launch over every problem's
+ * boxes.  Densities and values are the problems' vectors one after another
+ * (per right-hand side), in each problem's point order. */
+int kifmm_set_points_batch(kifmm_handle h, int nprob, const int64_t *n_src, const double *src_pos,
+                           const double *src_nor, const int64_t *n_trg, const double *trg_pos);
+
 /* KnlMat3d_MPI::setup() (knlmat3d_mpi.hpp:59; fmm3d_setup_mpi.cpp:41). */
 int kifmm_setup(kifmm_handle h);
 

@@ -434,3 +434,39 @@ def test_handle_reuse_new_points(gpu):
     assert torch.equal(outs[0], outs[1])
     ref = O.direct(111, [1.0, 0.25], b, np.zeros_like(b), den, b[:700])
     assert rel(outs[0].cpu().numpy(), ref) <= 1e-13
+
+
+def test_forest_navier_np8_vs_oracle():
+    """The many-problem workload, reduced: four Navier problems (np 8, ptsMax
+    120, maxLevel 10, nu 0.3) of uneven size in one handle (one launch per pass
+    over all trees), two right-hand sides through kifmm_evaluate_multi; every
+    problem's values against the oracle's evaluate of that problem alone, at
+    the FP64 bar."""
+    sizes = (2048, 5000, 900, 3300)
+    probs = [W.uniform(n, W.seed(4, 20 + p)) for p, n in enumerate(sizes)]
+    rng = np.random.default_rng(W.seed(4, 1))
+    S = sum(sizes)
+    den = rng.standard_normal((2, 3 * S))
+    print("case forest navier np8 sizes=%s nrhs=2" % (sizes,), flush=True)
+    f = kifmm.FMM3d(511, [1.0, 0.3], 8, 120, 10)
+    f.set_points_batch(probs)
+    f.setup()
+    vals = f.evaluate_multi(den)
+    nue, nuc = 8 ** 3 - 6 ** 3, 10 ** 3 - 8 ** 3
+    pu = f.operator(0).reshape(nuc * 3, nue * 3).T
+    pd = f.operator(1).reshape(nue * 3, nue * 3).T
+    ops = O.Ops(511, [1.0, 0.3], 8, pu, pd)
+    off = 0
+    for pos in probs:
+        n = len(pos)
+        tree = O.Tree(pos, None, pts_max=120, max_level=10)
+        for r in range(2):
+            ref = O.evaluate(tree, ops, 511, [1.0, 0.3], pos, np.zeros_like(pos), pos, den[r, 3 * off:3 * (off + n)])
+            assert rel(vals[r, 3 * off:3 * (off + n)], ref) <= PARITY
+        off += n
+    # the same forest in mixed precision against the FP64 forest
+    g = kifmm.FMM3d(511, [1.0, 0.3], 8, 120, 10, precision="mixed")
+    g.set_points_batch(probs)
+    g.setup()
+    mv = g.evaluate_multi(den)
+    assert rel(mv, vals) < 1e-6

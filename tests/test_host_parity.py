@@ -111,3 +111,38 @@ def test_work_counts_match_lists():
     u_pairs = sum(float(trg_num[b]) * src_num[u_lst[u_off[b]:u_off[b + 1]]].sum()
                   for b in range(len(trg_num)) if trg_num[b] > 0)
     assert w["near_exact_pairs"] >= u_pairs > 0
+
+
+def test_forest_is_the_per_problem_trees():
+    """Many problems in one handle (set_points_batch): the forest's nodes,
+    ranges, permutations and lists are each problem's own tree (the oracle's,
+    bit for bit) shifted by the node and point counts of the problems before
+    it; no list entry leaves its tree."""
+    probs = [W.uniform(3000, 501), W.gaussians(5000, 502), W.uniform(700, 503), W.gaussians(2200, 504)]
+    f = kifmm.FMM3d(511, [1.0, 0.3], 4, 40, 10)
+    f.set_points_batch(probs)
+    f.setup_host()
+    tr = f.tree()
+    node0 = pt0 = 0
+    for pos in probs:
+        ot = O.Tree(pos, None, pts_max=40, max_level=10)
+        sl = slice(node0, node0 + ot.n_nodes)
+        sh = lambda a, by: np.where(a >= 0, a + by, a)  # noqa: E731
+        assert np.array_equal(tr["parent"][sl], sh(ot.parent, node0))
+        assert np.array_equal(tr["child"][sl], sh(ot.child, node0))
+        for k in ["depth", "tag", "src_num", "trg_num"]:
+            assert np.array_equal(tr[k][sl], getattr(ot, k)), k
+        assert np.array_equal(tr["path"][sl], ot.path)
+        assert np.array_equal(tr["src_beg"][sl], sh(ot.src_beg, pt0))
+        assert np.array_equal(tr["trg_beg"][sl], sh(ot.trg_beg, pt0))
+        assert np.array_equal(tr["src_perm"][pt0:pt0 + len(pos)], ot.src_perm + pt0)
+        assert np.array_equal(tr["trg_perm"][pt0:pt0 + len(pos)], ot.trg_perm + pt0)
+        for name in "uvwx":
+            off, lst = tr["lists"][name]
+            ooff, olst = ot.lists[name]
+            got = lst[off[node0]:off[node0 + ot.n_nodes]]
+            assert np.array_equal(np.diff(off[node0:node0 + ot.n_nodes + 1]), np.diff(ooff)), name
+            assert np.array_equal(got, olst + node0), name
+        node0 += ot.n_nodes
+        pt0 += len(pos)
+    assert node0 == tr["parent"].size
