@@ -155,13 +155,7 @@ int kifmm_evaluate_multi_device(kifmm_handle h, const double* d_src_den, double*
                                 void* stream) {
   if (!h) return set_err(nullptr, KIFMM_ERR_ARG, "null handle");
   if (nrhs < 0) return set_err(h, KIFMM_ERR_ARG, "negative nrhs");
-  auto& f = h->fmm;
-  const int64_t nd = f.numSrc() * f.srcDOF(), nv = f.numTrg() * f.trgDOF();
-  for (int r = 0; r < nrhs; r++) {
-    int rc = f.evaluate(d_src_den + r * nd, d_trg_val + r * nv, static_cast<hipStream_t>(stream));
-    if (rc) return rc;
-  }
-  return KIFMM_OK;
+  return h->fmm.evaluateMulti(d_src_den, d_trg_val, nrhs, static_cast<hipStream_t>(stream));
 }
 
 int kifmm_evaluate_multi(kifmm_handle h, const double* src_den, double* trg_val, int nrhs) {

@@ -20,6 +20,10 @@ struct NearChunk {
   int own = -1, unchk = 0;
 };
 constexpr int kNearChunk = 128;
+// Right-hand sides per launch of the point kernels: groups of 1, 2, 4 or 8
+// (compile-time accumulator arrays); more go in several launches.
+constexpr int kMaxNr = 8;
+__host__ __device__ constexpr int rhs_group(int nr) { return nr <= 1 ? 1 : nr <= 2 ? 2 : nr <= 4 ? 4 : 8; }
 constexpr int kPairStride = 224;  // 216 halo slots rounded up to a multiple of 8
 // Source item code: (node << 2) | kind
 enum ItemKind { ITEM_EXACT = 0, ITEM_UE = 1, ITEM_DE = 2 };
@@ -44,6 +48,13 @@ struct NearArgs {
   // mixed Laplace near field: if set, value i (leaf order) goes to
   // tval[tperm[i]], the caller's order (no separate scatter)
   const int* tperm;
+  // Several right-hand sides: sden holds nrhs densities per source
+  // (sden[(k * nrhs + r) * SD + s]); right-hand side r's equivalent densities
+  // are rows r * nstride + node of UE and DE, and its values start at
+  // tval + r * tstride (tstride = n_trg * TD).  One launch covers the nr
+  // right-hand sides r0 .. r0 + nr - 1.
+  int nrhs = 1, r0 = 0, nr = 1;
+  int64_t nstride = 0, tstride = 0;
 };
 
 struct SurfArgs {
@@ -57,6 +68,10 @@ struct SurfArgs {
   // tasks of one leaf each, as (leaf, first source, source count): saves the
   // task -> item -> range lookups (mixed S2M; null elsewhere)
   const int4* leaf;
+  // several right-hand sides (as NearArgs): nrhs densities per source, right-
+  // hand side r writes rows r * nstride + node; this launch: r0 .. r0 + nr - 1
+  int nrhs = 1, r0 = 0, nr = 1;
+  int64_t nstride = 0;
 };
 
 struct GemmArgs {
@@ -89,8 +104,14 @@ struct VlistArgs {
   // with -1: source slot, and (halo index | offset slot of the halo index << 8)
   const int* psrc; const int* pslot; const int* pcnt;
   const double2* m2l;          // [343][sd*td][H]
-  double2* dspec;              // [nsrc][sd][H]
-  double2* espec;              // [ntrg][td][H]
+  double2* dspec;              // [nrhs][nsrc][sd][H]
+  double2* espec;              // [nrhs][ntrg][td][H]
+  // Several right-hand sides: right-hand side r of source slot s is spectrum
+  // row r * nsrc + s, read from UE row r * nstride + node (likewise targets
+  // and DC); the groups run as ngrp * nrhs virtual groups g * nrhs + r, so
+  // consecutive virtual groups share their translation spectra.
+  int nrhs = 1;
+  int64_t nstride = 0;
 };
 
 // Adjoint near field (source-centric): chunk = <= kNearChunk sources of a
@@ -130,6 +151,12 @@ hipError_t launch_surface_adjoint(int km, const SurfTArgs& a, const KParams<doub
 hipError_t launch_gather(const double* in, const int* perm, double* out, int64_t n, int dof, hipStream_t s);
 hipError_t launch_scatter(const double* in, const int* perm, double* out, int64_t n, int dof, hipStream_t s);
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s);
+// nrhs density vectors [r][k][dof] (caller's order) -> leaf order [sinv[k]][r][dof]
+hipError_t launch_load_rhs(const double* in, const int* sinv, double* out, int64_t n, int dof, int nrhs,
+                           hipStream_t s);
+// nrhs value vectors in leaf order [r][k][dof] -> caller's order [r][perm[k]][dof]
+hipError_t launch_store_rhs(const double* in, const int* perm, double* out, int64_t n, int dof, int nrhs,
+                            hipStream_t s);
 hipError_t launch_near(int kt, const NearArgs& a, const KParams<double>& pu, const KParams<double>& pm,
                        hipStream_t s);
 hipError_t launch_surface(int kt, const SurfArgs& a, const KParams<double>& pu, hipStream_t s);

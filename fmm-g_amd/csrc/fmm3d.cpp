@@ -50,6 +50,7 @@ void FMM3d::drop_graph() {
   graph_src_ = nullptr;
   graph_dst_ = nullptr;
   graph_stream_ = nullptr;
+  graph_nrhs_ = 0;
 }
 
 int FMM3d::srcDOF() const { return src_dof(kt_); }
@@ -147,6 +148,8 @@ int FMM3d::setupHost() {
   if (!ops_) return fail(-2, e);
   rc = build_tasks();
   if (rc) return rc;
+  rhs_steps_n_ = 1;
+  up_steps_r_.clear(); dn_steps_r_.clear(); lvl_idx_r_h_.clear();
   host_ready_ = true;
   return 0;
 }
@@ -507,6 +510,7 @@ int FMM3d::upload_state() {
   KIFMM_UP(depth_d_, t.depth, "upload depths");
   KIFMM_UP(sbeg_d_, t.src_beg, "upload ranges");
   KIFMM_UP(snum_d_, t.src_num, "upload ranges");
+  rhs_cap_ = 1;
   // per-node equivalent / check buffers
   const size_t n = size_t(N);
   if ((rc = check(ue_d_.alloc(sizeof(double) * n * op.dat(UE)), "alloc UE"))) return rc;
@@ -660,6 +664,7 @@ int FMM3d::run_vlist(VlistArgs& v, hipStream_t s, bool timed) {
 // ---------------------------------------------------------------------------
 int FMM3d::reserve_splitk(size_t doubles) {
   if (doubles * sizeof(double) <= splitk_d_.bytes()) return 0;
+  drop_graph();  // a captured pass holds the old workspace address
   return check(splitk_d_.alloc(doubles * sizeof(double)), "alloc split-K workspace");
 }
 
@@ -695,11 +700,30 @@ void split_k(GemmArgs& g) {
 int FMM3d::evaluate(const double* d_src_den, double* d_trg_val, hipStream_t s) {
   if (!setup_done_) return fail(-3, "evaluate called before setup");
   if (!use_graph_ || !s || timing_) return evaluate_impl(d_src_den, d_trg_val, s, false);
-  if (!(graph_exec_ && graph_src_ == d_src_den && graph_dst_ == d_trg_val && graph_stream_ == s)) {
+  return run_graph(d_src_den, d_trg_val, s, 1);
+}
+
+int FMM3d::evaluateMulti(const double* d_src_den, double* d_trg_val, int nrhs, hipStream_t s) {
+  if (!setup_done_) return fail(-3, "evaluate called before setup");
+  if (nrhs < 0) return fail(-2, "negative nrhs");
+  if (nrhs == 0) return 0;
+  if (nrhs == 1) return evaluate(d_src_den, d_trg_val, s);
+  int rc;
+  if ((rc = reserve_rhs(nrhs))) return rc;
+  if ((rc = prepare_rhs_steps(nrhs))) return rc;
+  if (!use_graph_ || !s || timing_) return evaluate_impl(d_src_den, d_trg_val, s, false, nrhs);
+  return run_graph(d_src_den, d_trg_val, s, nrhs);
+}
+
+// Replays the captured pass when (buffers, stream, nrhs) match the capture,
+// else captures anew.
+int FMM3d::run_graph(const double* d_src_den, double* d_trg_val, hipStream_t s, int nrhs) {
+  if (!(graph_exec_ && graph_src_ == d_src_den && graph_dst_ == d_trg_val && graph_stream_ == s &&
+        graph_nrhs_ == nrhs)) {
     drop_graph();
     int rc = check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin graph capture");
     if (rc) return rc;
-    rc = evaluate_impl(d_src_den, d_trg_val, s, false);
+    rc = evaluate_impl(d_src_den, d_trg_val, s, false, nrhs);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(s, &g);
     if (rc) {
@@ -715,8 +739,105 @@ int FMM3d::evaluate(const double* d_src_den, double* d_trg_val, hipStream_t s) {
     graph_src_ = d_src_den;
     graph_dst_ = d_trg_val;
     graph_stream_ = s;
+    graph_nrhs_ = nrhs;
   }
   return check(hipGraphLaunch(graph_exec_, s), "graph launch");
+}
+
+// Right-hand-side buffers for nrhs: node buffers, densities, values and
+// spectra hold nrhs blocks (allocated once per larger nrhs; a captured graph
+// holds the old addresses, so it is dropped).
+int FMM3d::reserve_rhs(int nrhs) {
+  if (nrhs <= rhs_cap_) return 0;
+  drop_graph();
+  const Operators& op = *ops_;
+  const size_t N = size_t(tree_.n_nodes), R = size_t(nrhs);
+  const int SD = srcDOF(), TD = trgDOF();
+  int rc;
+  auto grow = [&](DeviceBuffer& b, size_t bytes, const char* what) {
+    return b.bytes() >= bytes ? 0 : check(b.alloc(bytes), what);
+  };
+  if ((rc = grow(ue_d_, sizeof(double) * R * N * op.dat(UE), "alloc UE"))) return rc;
+  if ((rc = grow(uc_d_, sizeof(double) * R * N * op.dat(UC), "alloc UC"))) return rc;
+  if ((rc = grow(de_d_, sizeof(double) * R * N * op.dat(DE), "alloc DE"))) return rc;
+  if ((rc = grow(dc_d_, sizeof(double) * R * N * op.dat(DC), "alloc DC"))) return rc;
+  if ((rc = grow(sden_d_, sizeof(double) * R * std::max<int64_t>(1, ns_ * SD), "alloc densities"))) return rc;
+  if ((rc = grow(tval_d_, sizeof(double) * R * std::max<int64_t>(1, nt_ * TD), "alloc values"))) return rc;
+  const size_t nsv = vfwd_.src_node.size() * op.sd * R, ntv = vfwd_.trg_node.size() * op.td * R;
+  if (precision_ == PREC_FP64) {
+    if ((rc = grow(dspec_d_, sizeof(double2) * nsv * op.H, "alloc V spectra"))) return rc;
+    if ((rc = grow(espec_d_, sizeof(double2) * ntv * op.H, "alloc V spectra"))) return rc;
+  } else {
+    if ((rc = grow(dspec32_d_, sizeof(float2) * nsv * op.H, "alloc V spectra"))) return rc;
+    if ((rc = grow(espec32_d_, sizeof(float2) * ntv * op.H, "alloc V spectra"))) return rc;
+  }
+  rhs_cap_ = nrhs;
+  return 0;
+}
+
+// The level GEMM steps over nrhs right-hand sides: the columns of right-hand
+// side r are the single-RHS columns shifted to node rows r * n_nodes; the 8
+// L2L matrices keep their column blocks contiguous (matrix z: all right-hand
+// sides' columns of z).
+int FMM3d::prepare_rhs_steps(int nrhs) {
+  if (nrhs == rhs_steps_n_ && !up_steps_r_.empty()) return 0;
+  drop_graph();  // the column tables are uploaded anew
+  const int N = tree_.n_nodes;
+  lvl_idx_r_h_.clear();
+  auto expand = [&](const std::vector<GemmStep>& in, std::vector<GemmStep>& out) {
+    out.clear();
+    for (const GemmStep& st : in) {
+      GemmStep o = st;
+      const int* b = lvl_idx_h_.data() + st.idx_off;
+      const int* c = b + st.ncol;
+      o.ncol = st.ncol * nrhs;
+      o.idx_off = int64_t(lvl_idx_r_h_.size());
+      if (st.zoff_off < 0) {
+        for (const int* arr : {b, c})
+          for (int r = 0; r < nrhs; r++)
+            for (int k = 0; k < st.ncol; k++) lvl_idx_r_h_.push_back(arr[k] + r * N);
+        o.zoff_off = -1;
+        o.max_zcol = 0;
+      } else {
+        const int* zo = lvl_idx_h_.data() + st.zoff_off;
+        for (const int* arr : {b, c})
+          for (int z = 0; z < 8; z++)
+            for (int r = 0; r < nrhs; r++)
+              for (int k = zo[z]; k < zo[z + 1]; k++) lvl_idx_r_h_.push_back(arr[k] + r * N);
+        o.zoff_off = int64_t(lvl_idx_r_h_.size());
+        for (int z = 0; z < 9; z++) lvl_idx_r_h_.push_back(zo[z] * nrhs);
+        o.max_zcol = st.max_zcol * nrhs;
+      }
+      out.push_back(o);
+    }
+  };
+  expand(up_steps_, up_steps_r_);
+  expand(dn_steps_, dn_steps_r_);
+  int rc;
+  if ((rc = check(lvl_idx_r_d_.upload(lvl_idx_r_h_.data(), sizeof(int) * lvl_idx_r_h_.size()), "upload level columns")))
+    return rc;
+  if (precision_ == PREC_MIXED) {
+    const Operators& op = *ops_;
+    size_t need = 1;
+    for (const auto* steps : {&up_steps_r_, &dn_steps_r_})
+      for (const GemmStep& st : *steps) {
+        GemmArgs g{};
+        switch (st.mat) {
+          case 0: g.M = op.dat(UC); g.K = 8 * op.dat(UE); break;
+          case 1: g.M = op.dat(UE); g.K = op.dat(UC); break;
+          case 8: g.M = op.dat(UE); g.K = 8 * op.dat(UE); break;
+          case 2: g.M = op.dat(DE); g.K = op.dat(DC); break;
+          default: g.M = op.dat(DC); g.K = op.dat(DE);
+        }
+        g.ncol = st.ncol; g.max_zcol = st.ncol;
+        if (st.zoff_off >= 0) { g.zoff = &st.ncol; g.nz = 8; g.max_zcol = st.max_zcol; }
+        split_k(g);
+        if (g.ksplit > 1) need = std::max(need, size_t(g.ksplit) * size_t(g.ncol) * size_t(g.M));
+      }
+    if ((rc = reserve_splitk(need))) return rc;
+  }
+  rhs_steps_n_ = nrhs;
+  return 0;
 }
 
 int FMM3d::field(const double* d_src_den, double* d_trg_grad, hipStream_t s) {
@@ -731,8 +852,10 @@ int FMM3d::field(const double* d_src_den, double* d_trg_grad, hipStream_t s) {
 
 // grad: the same far-field pass, then the gradient of the near-field and local
 // expansion kernels at the targets (3 values per target) instead of the values.
-int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t s, bool grad) {
+int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t s, bool grad, int nrhs) {
   if ((ns_ > 0 && !d_src_den) || (nt_ > 0 && !d_trg_val)) return fail(-2, "null density or value pointer");
+  const int R = nrhs;
+  if (R < 1 || R > rhs_cap_ || (R > 1 && (grad || rhs_steps_n_ != R))) return fail(-6, "right-hand sides not prepared");
   const Tree& t = tree_;
   const Operators& op = *ops_;
   const int N = t.n_nodes, SD = srcDOF(), TD = trgDOF();
@@ -745,17 +868,19 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
 
   mark(PH_INI);
   // zero the work vectors (:425-438)
-  if ((rc = check(launch_fill(ue_d_.as<double>(), int64_t(N) * op.dat(UE), 0.0, s), "zero UE"))) return rc;
+  const int64_t NR = int64_t(N) * R;  // node rows over the right-hand sides
+  if ((rc = check(launch_fill(ue_d_.as<double>(), NR * op.dat(UE), 0.0, s), "zero UE"))) return rc;
   // mixed path: UC is read only for the contributor leaves, whose S2M task
   // stores it whole (the fused up pass forms no UC for parents)
-  if (!mixed && (rc = check(launch_fill(uc_d_.as<double>(), int64_t(N) * op.dat(UC), 0.0, s), "zero UC")))
+  if (!mixed && (rc = check(launch_fill(uc_d_.as<double>(), NR * op.dat(UC), 0.0, s), "zero UC")))
     return rc;
   // DE: the downward pseudo-inverse stores it for every evaluator node at depth
   // >= 2, and only those rows are read (L2L, L2T); the mixed path skips the fill
-  if (!mixed && (rc = check(launch_fill(de_d_.as<double>(), int64_t(N) * op.dat(DE), 0.0, s), "zero DE")))
+  if (!mixed && (rc = check(launch_fill(de_d_.as<double>(), NR * op.dat(DE), 0.0, s), "zero DE")))
     return rc;
-  if ((rc = check(launch_fill(dc_d_.as<double>(), int64_t(N) * op.dat(DC), 0.0, s), "zero DC"))) return rc;
-  if ((rc = check(launch_scatter(d_src_den, sinv_d_.as<int>(), sden_d_.as<double>(), ns_, SD, s), "load densities")))
+  if ((rc = check(launch_fill(dc_d_.as<double>(), NR * op.dat(DC), 0.0, s), "zero DC"))) return rc;
+  if ((rc = check(launch_load_rhs(d_src_den, sinv_d_.as<int>(), sden_d_.as<double>(), ns_, SD, R, s),
+                  "load densities")))
     return rc;
 
   SurfArgs sa{};
@@ -763,9 +888,12 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
   sa.spos = spos_d_.as<double>(); sa.snor = snor_d_.as<double>(); sa.sden = sden_d_.as<double>();
   sa.src_beg = sbeg_d_.as<int>(); sa.src_num = snum_d_.as<int>();
   sa.geom = geom_d_.as<double4>();
+  sa.nrhs = R;
+  sa.nstride = N;
 
   double* bufs[4] = {ue_d_.as<double>(), uc_d_.as<double>(), de_d_.as<double>(), dc_d_.as<double>()};
   const int szs[4] = {op.dat(UE), op.dat(UC), op.dat(DE), op.dat(DC)};
+  const int* lvl_idx = R > 1 ? lvl_idx_r_d_.as<int>() : lvl_idx_d_.as<int>();
   auto run_steps = [&](const std::vector<GemmStep>& steps) -> int {
     for (const GemmStep& st : steps) {
       GemmArgs g{};
@@ -780,15 +908,15 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
       }
       g.max_zcol = st.ncol;
       if (st.zoff_off >= 0) {
-        g.zoff = lvl_idx_d_.as<int>() + st.zoff_off;
+        g.zoff = lvl_idx + st.zoff_off;
         g.nz = 8;
         g.a_zstride = int64_t(op.dat(DC)) * op.dat(DE);
         g.max_zcol = st.max_zcol;
       }
       g.B = bufs[st.src]; g.ldb = szs[st.src];
       g.C = bufs[st.dst]; g.ldc = szs[st.dst];
-      g.bidx = lvl_idx_d_.as<int>() + st.idx_off;
-      g.cidx = lvl_idx_d_.as<int>() + st.idx_off + st.ncol;
+      g.bidx = lvl_idx + st.idx_off;
+      g.cidx = lvl_idx + st.idx_off + st.ncol;
       if (st.ks_off >= 0) { g.kscale = scales_d_.as<double>() + st.ks_off; g.ksd = op.sd; }
       else { g.kscale = ones_d_.as<double>(); g.ksd = 1; }
       if (st.rs_off >= 0) { g.rscale = scales_d_.as<double>() + st.rs_off; g.rsd = op.sd; }
@@ -815,7 +943,7 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
   sa.sam = sam_d_[UC].as<double>(); sa.n_sam = op.n_sam[UC];
   sa.out = uc_d_.as<double>(); sa.sz_out = op.dat(UC);
   if ((rc = check(mixed ? launch_surface_mixed(kt_, sa, pu, s) : launch_surface(kt_, sa, pu, s), "S2M"))) return rc;
-  if ((rc = run_steps(up_steps_))) return rc;
+  if ((rc = run_steps(R > 1 ? up_steps_r_ : up_steps_))) return rc;
 
   // V list (:1007-1059)
   mark(PH_VLIST);
@@ -823,6 +951,8 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
   vlist_args(v, vfwd_);
   v.UE = ue_d_.as<double>(); v.sz_ue = op.dat(UE);
   v.DC = dc_d_.as<double>(); v.sz_dc = op.dat(DC);
+  v.nrhs = R;
+  v.nstride = N;
   if ((rc = run_vlist(v, s, true))) return rc;
 
   // X list S2L (:1233-1265)
@@ -835,7 +965,7 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
 
   // down pass (:1267-1382)
   mark(PH_DOWN);
-  if ((rc = run_steps(dn_steps_))) return rc;
+  if ((rc = run_steps(R > 1 ? dn_steps_r_ : dn_steps_))) return rc;
 
   // U, W, X-fallback and L2T at the targets (:653-794, :1062-1231, :1383-1435)
   mark(PH_NEAR);
@@ -851,12 +981,15 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
   na.UE = ue_d_.as<double>(); na.sz_ue = op.dat(UE);
   na.DE = de_d_.as<double>(); na.sz_de = op.dat(DE);
   na.tpos = tpos_d_.as<double>(); na.tval = tval_d_.as<double>();
+  na.nrhs = R;
+  na.nstride = N;
+  na.tstride = nt_ * TD;
   if (grad) {
     na.tval = tgrad_d_.as<double>();
     if ((rc = check(launch_near_grad(kt_, na, pu, pm, s), "near field gradient"))) return rc;
   } else {
     // mixed Laplace: the near kernel writes the final values in the caller's order
-    if (fused_out) { na.tval = d_trg_val; na.tperm = tperm_d_.as<int>(); }
+    if (fused_out) { na.tval = d_trg_val; na.tperm = tperm_d_.as<int>(); na.tstride = nt_; }
     if ((rc = check(mixed ? launch_near_mixed(kt_, na, pu, pm, s) : launch_near(kt_, na, pu, pm, s), "near field")))
       return rc;
   }
@@ -864,8 +997,8 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
   // write back (:1490-1510)
   mark(PH_FIN);
   if (!fused_out &&
-      (rc = check(launch_scatter(grad ? tgrad_d_.as<double>() : tval_d_.as<double>(), tperm_d_.as<int>(), d_trg_val,
-                                 nt_, grad ? 3 : TD, s), "scatter")))
+      (rc = check(launch_store_rhs(grad ? tgrad_d_.as<double>() : tval_d_.as<double>(), tperm_d_.as<int>(), d_trg_val,
+                                   nt_, grad ? 3 : TD, R, s), "scatter")))
     return rc;
   mark(PH_COUNT);
   if (timing_) {

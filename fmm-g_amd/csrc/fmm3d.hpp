@@ -91,6 +91,13 @@ class FMM3d {
   int setup();
   // Device pointers, caller's point order (srcDOF / trgDOF interleaved).
   int evaluate(const double* d_src_den, double* d_trg_val, hipStream_t stream);
+  // nrhs right-hand sides in one pass: d_src_den holds nrhs density vectors and
+  // d_trg_val receives nrhs value vectors, one after another.  Every pass runs
+  // one launch for all of them (the point kernels stage each source once for
+  // all right-hand sides, the GEMMs take nrhs times the columns, the V list
+  // nrhs times the boxes); the right-hand-side buffers grow on the first call
+  // with a larger nrhs.
+  int evaluateMulti(const double* d_src_den, double* d_trg_val, int nrhs, hipStream_t stream);
   // Transposed pass: d_src_out (srcDOF per source) = A^T d_trg_in (trgDOF per
   // target), A the evaluate operator.  Needs a symmetric equivalent kernel
   // (111 or 511); the adjoint tables are built on the first call.
@@ -132,7 +139,10 @@ class FMM3d {
  private:
   int fail(int code, const std::string& msg) { err_ = msg; return code; }
   int check(hipError_t e, const char* what);
-  int evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t s, bool grad);
+  int evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t s, bool grad, int nrhs = 1);
+  int run_graph(const double* d_src_den, double* d_trg_val, hipStream_t s, int nrhs);
+  int reserve_rhs(int nrhs);
+  int prepare_rhs_steps(int nrhs);
   void drop_graph();
   int build_tasks();
   int upload_state();
@@ -171,6 +181,12 @@ class FMM3d {
     int max_zcol = 0;
   };
   std::vector<GemmStep> up_steps_, dn_steps_;
+  // the same steps over nrhs right-hand sides (columns of right-hand side r
+  // address node rows r * n_nodes + node), built for rhs_steps_n_
+  std::vector<GemmStep> up_steps_r_, dn_steps_r_;
+  std::vector<int> lvl_idx_r_h_;
+  DeviceBuffer lvl_idx_r_d_;
+  int rhs_steps_n_ = 1, rhs_cap_ = 1;
   DeviceBuffer splitk_d_;  // mixed path: split-K partial products (ordered sum, no atomics)
   int reserve_splitk(size_t doubles);
   double* splitk_for(const GemmArgs& g);
@@ -214,6 +230,7 @@ class FMM3d {
   const double* graph_src_ = nullptr;
   double* graph_dst_ = nullptr;
   hipStream_t graph_stream_ = nullptr;
+  int graph_nrhs_ = 0;
 };
 
 }  // namespace kifmm

@@ -176,7 +176,9 @@ int FMM3d::build_adjoint() {
   }
 #undef KIFMM_UP
   if ((rc = upload_vgroups(vadj_))) return rc;
-  // spectra buffers hold the larger of the two passes
+  // spectra buffers hold the larger of the two passes (a captured forward pass
+  // holds the old addresses)
+  drop_graph();
   {
     const size_t nsrc = std::max(vfwd_.src_node.size(), vadj_.src_node.size()) * op.sd;
     const size_t ntrg = std::max(vfwd_.trg_node.size(), vadj_.trg_node.size()) * op.td;

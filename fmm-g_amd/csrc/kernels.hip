@@ -35,6 +35,25 @@ __global__ void k_scatter(const double* __restrict__ in, const int* __restrict__
   for (int d = 0; d < dof; d++) out[o * dof + d] = in[k * dof + d];
 }
 
+// Densities of nrhs right-hand sides, caller's layout [r][k][dof] -> leaf order
+// [sinv[k]][r][dof] (the right-hand sides of a source side by side).
+__global__ void k_load_rhs(const double* __restrict__ in, const int* __restrict__ sinv, double* __restrict__ out,
+                           int64_t n, int dof, int nrhs) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n * nrhs) return;
+  const int64_t r = i / n, k = i % n, o = sinv[k];
+  for (int d = 0; d < dof; d++) out[(o * nrhs + r) * dof + d] = in[i * dof + d];
+}
+
+// Values of nrhs right-hand sides, leaf order [r][k][dof] -> caller's [r][perm[k]][dof].
+__global__ void k_store_rhs(const double* __restrict__ in, const int* __restrict__ perm, double* __restrict__ out,
+                            int64_t n, int dof, int nrhs) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n * nrhs) return;
+  const int64_t r = i / n, k = i % n, o = r * n + perm[k];
+  for (int d = 0; d < dof; d++) out[o * dof + d] = in[i * dof + d];
+}
+
 __global__ void k_fill(double* p, int64_t n, double v) {
   int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   for (; i < n; i += int64_t(gridDim.x) * blockDim.x) p[i] = v;
@@ -48,6 +67,18 @@ hipError_t launch_gather(const double* in, const int* perm, double* out, int64_t
 hipError_t launch_scatter(const double* in, const int* perm, double* out, int64_t n, int dof, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   k_scatter<<<dim3(unsigned((n + 255) / 256)), dim3(256), 0, s>>>(in, perm, out, n, dof);
+  return hipGetLastError();
+}
+hipError_t launch_load_rhs(const double* in, const int* sinv, double* out, int64_t n, int dof, int nrhs,
+                           hipStream_t s) {
+  if (n <= 0 || nrhs <= 0) return hipSuccess;
+  k_load_rhs<<<dim3(unsigned((n * nrhs + 255) / 256)), dim3(256), 0, s>>>(in, sinv, out, n, dof, nrhs);
+  return hipGetLastError();
+}
+hipError_t launch_store_rhs(const double* in, const int* perm, double* out, int64_t n, int dof, int nrhs,
+                            hipStream_t s) {
+  if (n <= 0 || nrhs <= 0) return hipSuccess;
+  k_store_rhs<<<dim3(unsigned((n * nrhs + 255) / 256)), dim3(256), 0, s>>>(in, perm, out, n, dof, nrhs);
   return hipGetLastError();
 }
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s) {
@@ -67,25 +98,31 @@ hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s) {
 // pseudo-inverse, so its summation order is free; for the Laplace kernel 1/r
 // comes from rsqrt.
 // ---------------------------------------------------------------------------
-template <int KT>
+template <int KT, int NR>
 __device__ inline void near_pair(const KParams<double>& p, double dx, double dy, double dz, double nx,
-                                 double ny, double nz, const double* den, double* acc) {
+                                 double ny, double nz, const double* den, double (*acc)[trg_dof(KT)]) {
   constexpr int SD = src_dof(KT), TD = trg_dof(KT);
   if constexpr (KT == 111) {
     const double r2 = dx * dx + dy * dy + dz * dz;
     const double g = r2 < kMinDif * kMinDif ? 0.0 : p.a * rsqrt(r2);
-    acc[0] += g * den[0];
+#pragma unroll
+    for (int r = 0; r < NR; r++) acc[r][0] += g * den[r];
   } else {
     double blk[SD * TD];
     kblock<KT, double>(p, dx, dy, dz, nx, ny, nz, blk);
 #pragma unroll
-    for (int t = 0; t < TD; t++)
+    for (int r = 0; r < NR; r++)
 #pragma unroll
-      for (int s = 0; s < SD; s++) acc[t] += blk[t * SD + s] * den[s];
+      for (int t = 0; t < TD; t++)
+#pragma unroll
+        for (int s = 0; s < SD; s++) acc[r][t] += blk[t * SD + s] * den[r * SD + s];
   }
 }
 
-template <int KT>
+// NR: right-hand sides per launch (the kernel value of a pair is formed once and
+// applied to all of them); each right-hand side's sums run in the single-RHS
+// order, so its values are bitwise those of a one-RHS launch.
+template <int KT, int NR>
 __global__ __launch_bounds__(kNearChunk) void k_near(NearArgs a, KParams<double> pu, KParams<double> pm) {
   constexpr int KM = mm_kernel(KT);
   constexpr int SD = src_dof(KT), TD = trg_dof(KT), sd = src_dof(KM);
@@ -94,7 +131,7 @@ __global__ __launch_bounds__(kNearChunk) void k_near(NearArgs a, KParams<double>
   static_assert(trg_dof(KM) == TD, "user and equivalent kernels share the target DOF");
   __shared__ double sp[kNearChunk * 3];
   __shared__ double sn[NOR ? kNearChunk * 3 : 1];
-  __shared__ double sv[kNearChunk * MAXD];
+  __shared__ double sv[kNearChunk * MAXD * NR];
   __shared__ double red[kNearChunk * TD];
   const NearChunk ch = a.chunks[blockIdx.x];
   const int lane = threadIdx.x;
@@ -107,9 +144,11 @@ __global__ __launch_bounds__(kNearChunk) void k_near(NearArgs a, KParams<double>
     const double* p = a.tpos + 3 * int64_t(ch.tbeg + ti);
     tx = p[0]; ty = p[1]; tz = p[2];
   }
-  double acc[TD];
+  double acc[NR][TD];
 #pragma unroll
-  for (int t = 0; t < TD; t++) acc[t] = 0.0;
+  for (int r = 0; r < NR; r++)
+#pragma unroll
+    for (int t = 0; t < TD; t++) acc[r][t] = 0.0;
 
   for (int it = ch.ibeg; it < ch.iend; it++) {
     const int code = a.items[it];
@@ -123,15 +162,18 @@ __global__ __launch_bounds__(kNearChunk) void k_near(NearArgs a, KParams<double>
           const int64_t g = int64_t(b + base + lane);
           for (int d = 0; d < 3; d++) sp[3 * lane + d] = a.spos[3 * g + d];
           if constexpr (NOR) for (int d = 0; d < 3; d++) sn[3 * lane + d] = a.snor[3 * g + d];
-          for (int s = 0; s < SD; s++) sv[lane * SD + s] = a.sden[g * SD + s];
+#pragma unroll
+          for (int r = 0; r < NR; r++)
+            for (int s = 0; s < SD; s++)
+              sv[(lane * NR + r) * SD + s] = r < a.nr ? a.sden[(g * a.nrhs + a.r0 + r) * SD + s] : 0.0;
         }
         __syncthreads();
         if (active) {
           for (int j = sub; j < m; j += S) {
             double nx = 0, ny = 0, nz = 0;
             if constexpr (NOR) { nx = sn[3 * j]; ny = sn[3 * j + 1]; nz = sn[3 * j + 2]; }
-            near_pair<KT>(pu, tx - sp[3 * j], ty - sp[3 * j + 1], tz - sp[3 * j + 2], nx, ny, nz,
-                          &sv[j * SD], acc);
+            near_pair<KT, NR>(pu, tx - sp[3 * j], ty - sp[3 * j + 1], tz - sp[3 * j + 2], nx, ny, nz,
+                              &sv[j * NR * SD], acc);
           }
         }
       }
@@ -141,6 +183,7 @@ __global__ __launch_bounds__(kNearChunk) void k_near(NearArgs a, KParams<double>
       const double* sam = ue ? a.sam_ue : a.sam_de;
       const int ns = ue ? a.n_ue : a.n_de;
       const double* den = ue ? a.UE + int64_t(node) * a.sz_ue : a.DE + int64_t(node) * a.sz_de;
+      const int64_t rstep = a.nstride * (ue ? a.sz_ue : a.sz_de);
       for (int base = 0; base < ns; base += kNearChunk) {
         const int m = min(kNearChunk, ns - base);
         __syncthreads();
@@ -149,29 +192,38 @@ __global__ __launch_bounds__(kNearChunk) void k_near(NearArgs a, KParams<double>
           sp[3 * lane] = g.x + g.w * sam[3 * q];
           sp[3 * lane + 1] = g.y + g.w * sam[3 * q + 1];
           sp[3 * lane + 2] = g.z + g.w * sam[3 * q + 2];
-          for (int s = 0; s < sd; s++) sv[lane * sd + s] = den[q * sd + s];
+#pragma unroll
+          for (int r = 0; r < NR; r++)
+            for (int s = 0; s < sd; s++)
+              sv[(lane * NR + r) * sd + s] = r < a.nr ? den[(a.r0 + r) * rstep + q * sd + s] : 0.0;
         }
         __syncthreads();
         if (active) {
           for (int j = sub; j < m; j += S) {
             const double x = sp[3 * j], y = sp[3 * j + 1], z = sp[3 * j + 2];
-            near_pair<KM>(pm, tx - x, ty - y, tz - z, x, y, z, &sv[j * sd], acc);
+            near_pair<KM, NR>(pm, tx - x, ty - y, tz - z, x, y, z, &sv[j * NR * sd], acc);
           }
         }
       }
     }
   }
-  // reduce the S partial sums of each target
-  __syncthreads();
-  if (active && sub > 0)
-    for (int t = 0; t < TD; t++) red[(lane - ch.tnum) * TD + t] = acc[t];
-  __syncthreads();
-  if (lane < ch.tnum) {
-    for (int u = 1; u < S; u++)
-      for (int t = 0; t < TD; t++) acc[t] += red[((u - 1) * ch.tnum + lane) * TD + t];
-    double* o = a.tval + int64_t(ch.tbeg + lane) * TD;
+  // reduce the S partial sums of each target, one right-hand side at a time
 #pragma unroll
-    for (int t = 0; t < TD; t++) o[t] = acc[t];
+  for (int r = 0; r < NR; r++) {
+    __syncthreads();
+    if (active && sub > 0)
+      for (int t = 0; t < TD; t++) red[(lane - ch.tnum) * TD + t] = acc[r][t];
+    __syncthreads();
+    if (lane < ch.tnum && r < a.nr) {
+      double v[TD];
+#pragma unroll
+      for (int t = 0; t < TD; t++) v[t] = acc[r][t];
+      for (int u = 1; u < S; u++)
+        for (int t = 0; t < TD; t++) v[t] += red[((u - 1) * ch.tnum + lane) * TD + t];
+      double* o = a.tval + int64_t(a.r0 + r) * a.tstride + int64_t(ch.tbeg + lane) * TD;
+#pragma unroll
+      for (int t = 0; t < TD; t++) o[t] = v[t];
+    }
   }
 }
 
@@ -181,17 +233,18 @@ __global__ __launch_bounds__(kNearChunk) void k_near(NearArgs a, KParams<double>
 // ---------------------------------------------------------------------------
 constexpr int kSurfThreads = 256;
 
-template <int KT>
+template <int KT, int NR>
 __global__ __launch_bounds__(kSurfThreads) void k_surface(SurfArgs a, KParams<double> pu) {
   constexpr int SD = src_dof(KT), TD = trg_dof(KT);
   __shared__ double sp[kSurfThreads * 3];
   __shared__ double sn[kSurfThreads * 3];
-  __shared__ double sv[kSurfThreads * SD];
+  __shared__ double sv[kSurfThreads * SD * NR];
   const SurfTask tk = a.tasks[blockIdx.x];
   const double4 g = a.geom[tk.node];
   const int lane = threadIdx.x;
+  const int64_t rstep = a.nstride * a.sz_out;
   double tp[2][3];
-  double acc[2][TD];
+  double acc[2][NR][TD];
   bool act[2];
   for (int u = 0; u < 2; u++) {
     const int q = lane + u * kSurfThreads;
@@ -201,7 +254,11 @@ __global__ __launch_bounds__(kSurfThreads) void k_surface(SurfArgs a, KParams<do
     tp[u][1] = g.y + g.w * a.sam[3 * qq + 1];
     tp[u][2] = g.z + g.w * a.sam[3 * qq + 2];
     // start from the current check value and add term by term (dgemv order)
-    for (int t = 0; t < TD; t++) acc[u][t] = act[u] ? a.out[int64_t(tk.node) * a.sz_out + q * TD + t] : 0.0;
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+      for (int t = 0; t < TD; t++)
+        acc[u][r][t] = act[u] && r < a.nr
+                           ? a.out[(a.r0 + r) * rstep + int64_t(tk.node) * a.sz_out + q * TD + t] : 0.0;
   }
   for (int it = tk.ibeg; it < tk.iend; it++) {
     const int node = a.items[it];
@@ -212,7 +269,10 @@ __global__ __launch_bounds__(kSurfThreads) void k_surface(SurfArgs a, KParams<do
       if (lane < m) {
         const int64_t gg = int64_t(b + base + lane);
         for (int d = 0; d < 3; d++) { sp[3 * lane + d] = a.spos[3 * gg + d]; sn[3 * lane + d] = a.snor[3 * gg + d]; }
-        for (int s = 0; s < SD; s++) sv[lane * SD + s] = a.sden[gg * SD + s];
+#pragma unroll
+        for (int r = 0; r < NR; r++)
+          for (int s = 0; s < SD; s++)
+            sv[(lane * NR + r) * SD + s] = r < a.nr ? a.sden[(gg * a.nrhs + a.r0 + r) * SD + s] : 0.0;
       }
       __syncthreads();
       for (int j = 0; j < m; j++) {
@@ -223,18 +283,24 @@ __global__ __launch_bounds__(kSurfThreads) void k_surface(SurfArgs a, KParams<do
           kblock<KT, double>(pu, tp[u][0] - sp[3 * j], tp[u][1] - sp[3 * j + 1], tp[u][2] - sp[3 * j + 2],
                              sn[3 * j], sn[3 * j + 1], sn[3 * j + 2], blk);
 #pragma unroll
-          for (int t = 0; t < TD; t++)
+          for (int r = 0; r < NR; r++)
 #pragma unroll
-            for (int s = 0; s < SD; s++) acc[u][t] += blk[t * SD + s] * sv[j * SD + s];
+            for (int t = 0; t < TD; t++)
+#pragma unroll
+              for (int s = 0; s < SD; s++) acc[u][r][t] += blk[t * SD + s] * sv[(j * NR + r) * SD + s];
         }
       }
     }
   }
-  double* o = a.out + int64_t(tk.node) * a.sz_out;
   for (int u = 0; u < 2; u++) {
     if (!act[u]) continue;
     const int q = lane + u * kSurfThreads;
-    for (int t = 0; t < TD; t++) o[q * TD + t] = acc[u][t];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      if (r >= a.nr) continue;
+      double* o = a.out + (a.r0 + r) * rstep + int64_t(tk.node) * a.sz_out;
+      for (int t = 0; t < TD; t++) o[q * TD + t] = acc[u][r][t];
+    }
   }
 }
 
@@ -337,8 +403,9 @@ __global__ __launch_bounds__(64) void k_fft_fwd(VlistArgs v, C2* dspec) {
   __shared__ double xb[NP * NP * NP];
   __shared__ LT t1[NP * NP * HH];
   __shared__ LT t2[NP * N * HH];
-  const int slot = blockIdx.x;
-  const int node = v.src_node[slot];
+  const int slot = blockIdx.x;  // right-hand side slot / nsrc, source slot % nsrc
+  const int node = v.src_node[slot % v.nsrc];
+  const int64_t urow = int64_t(node) + int64_t(slot / v.nsrc) * v.nstride;
   const int lane = threadIdx.x;
   double2 w[N];  // exp(-2 pi i m / n)
 #pragma unroll
@@ -358,7 +425,7 @@ __global__ __launch_bounds__(64) void k_fft_fwd(VlistArgs v, C2* dspec) {
       const bool ok = q < NSAM;
       const int qq = ok ? q : 0;
       sh[r] = v.shell[qq];
-      const double u = v.UE[int64_t(node) * v.sz_ue + qq * v.sd + s];
+      const double u = v.UE[urow * v.sz_ue + qq * v.sd + s];
       uv[r] = ok ? u * scale : 0.0;
     }
     __syncthreads();
@@ -440,7 +507,8 @@ __global__ __launch_bounds__(64) void k_fft_fwd(VlistArgs v, C2* dspec) {
 // (UpwEqu2DwnChk_dgemv + cptwvv).
 template <int SD, int TD>
 __global__ __launch_bounds__(64) void k_hadamard(VlistArgs v, double nrm) {
-  const int g = blockIdx.x;
+  const int g = blockIdx.x / v.nrhs, rh = blockIdx.x % v.nrhs;  // virtual group g * nrhs + rh
+  const int soff = rh * v.nsrc, toff = rh * v.ntrg;
   const int f = blockIdx.y * 64 + threadIdx.x;
   const int H = v.H;
   const bool ok = f < H;
@@ -458,7 +526,7 @@ __global__ __launch_bounds__(64) void k_hadamard(VlistArgs v, double nrm) {
     if (s < 0) continue;
     double2 d[SD];
 #pragma unroll
-    for (int j = 0; j < SD; j++) d[j] = v.dspec[(int64_t(s) * SD + j) * H + ff];
+    for (int j = 0; j < SD; j++) d[j] = v.dspec[(int64_t(s + soff) * SD + j) * H + ff];
     const int rs = (word >> 8) & 0x3ff, tm = word >> 20;
 #pragma unroll
     for (int tau = 0; tau < 8; tau++) {
@@ -480,7 +548,7 @@ __global__ __launch_bounds__(64) void k_hadamard(VlistArgs v, double nrm) {
     const int t = v.tgt[g * 8 + tau];
     if (t < 0) continue;
 #pragma unroll
-    for (int i = 0; i < TD; i++) v.espec[(int64_t(t) * TD + i) * H + f] = acc[tau][i];
+    for (int i = 0; i < TD; i++) v.espec[(int64_t(t + toff) * TD + i) * H + f] = acc[tau][i];
   }
 }
 
@@ -497,8 +565,9 @@ __global__ __launch_bounds__(64) void k_fft_inv(VlistArgs v, const C2* espec) {
   __shared__ LT y1[NP * N * HH];
   __shared__ LT y2[NP * NP * HH];
   __shared__ int qof[NP * NP * NP];  // block index -> surface sample, -1 inside
-  const int slot = blockIdx.x;
-  const int node = v.trg_node[slot];
+  const int slot = blockIdx.x;  // right-hand side slot / ntrg, target slot % ntrg
+  const int node = v.trg_node[slot % v.ntrg];
+  const int64_t drow = int64_t(node) + int64_t(slot / v.ntrg) * v.nstride;
   const int lane = threadIdx.x;
   double2 w[N];  // exp(+2 pi i m / n)
 #pragma unroll
@@ -506,7 +575,7 @@ __global__ __launch_bounds__(64) void k_fft_inv(VlistArgs v, const C2* espec) {
   for (int e = lane; e < NP * NP * NP; e += 64) qof[e] = -1;
   __syncthreads();
   for (int q = lane; q < NSAM; q += 64) qof[v.shell[q]] = q;
-  double* dc = v.DC + int64_t(node) * v.sz_dc;
+  double* dc = v.DC + drow * v.sz_dc;
   for (int i = 0; i < v.td; i++) {
     const C2* in = espec + (int64_t(slot) * v.td + i) * (N * N * HH);
     __syncthreads();
@@ -611,7 +680,7 @@ __global__ __launch_bounds__(64) void k_fft_inv(VlistArgs v, const C2* espec) {
 
 template <typename C2>
 static hipError_t launch_fft_fwd_any(const VlistArgs& v, C2* dspec, hipStream_t s) {
-  const dim3 g(unsigned(v.nsrc)), b(64);
+  const dim3 g(unsigned(int64_t(v.nsrc) * v.nrhs)), b(64);
   switch (v.np) {
     case 3: k_fft_fwd<3, C2><<<g, b, 0, s>>>(v, dspec); break;
     case 4: k_fft_fwd<4, C2><<<g, b, 0, s>>>(v, dspec); break;
@@ -626,7 +695,7 @@ static hipError_t launch_fft_fwd_any(const VlistArgs& v, C2* dspec, hipStream_t 
 
 template <typename C2>
 static hipError_t launch_fft_inv_any(const VlistArgs& v, const C2* espec, hipStream_t s) {
-  const dim3 g(unsigned(v.ntrg)), b(64);
+  const dim3 g(unsigned(int64_t(v.ntrg) * v.nrhs)), b(64);
   switch (v.np) {
     case 3: k_fft_inv<3, C2><<<g, b, 0, s>>>(v, espec); break;
     case 4: k_fft_inv<4, C2><<<g, b, 0, s>>>(v, espec); break;
@@ -641,12 +710,12 @@ static hipError_t launch_fft_inv_any(const VlistArgs& v, const C2* espec, hipStr
 
 hipError_t launch_vlist(const VlistArgs& v, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   if (v.ngrp <= 0) return hipSuccess;
-  if (v.np > kMaxNp) return hipErrorInvalidValue;
+  if (v.np > kMaxNp || v.nrhs < 1) return hipErrorInvalidValue;
   hipError_t e = launch_fft_fwd_any<double2>(v, v.dspec, s);
   if (e != hipSuccess) return e;
   if (e0) (void)hipEventRecord(e0, s);
   const double nrm = 1.0 / double(v.n * v.n * v.n);
-  dim3 grid(unsigned(v.ngrp), unsigned((v.H + 63) / 64));
+  dim3 grid(unsigned(int64_t(v.ngrp) * v.nrhs), unsigned((v.H + 63) / 64));
   if (v.sd == 1 && v.td == 1) k_hadamard<1, 1><<<grid, dim3(64), 0, s>>>(v, nrm);
   else if (v.sd == 4 && v.td == 3) k_hadamard<4, 3><<<grid, dim3(64), 0, s>>>(v, nrm);
   else if (v.sd == 3 && v.td == 3) k_hadamard<3, 3><<<grid, dim3(64), 0, s>>>(v, nrm);
@@ -659,19 +728,16 @@ hipError_t launch_vlist(const VlistArgs& v, hipStream_t s, hipEvent_t e0, hipEve
 
 hipError_t launch_fft_fwd32(const VlistArgs& v, float2* dspec, hipStream_t s) {
   if (v.nsrc <= 0) return hipSuccess;
-  if (v.np > kMaxNp) return hipErrorInvalidValue;
+  if (v.np > kMaxNp || v.nrhs < 1) return hipErrorInvalidValue;
   return launch_fft_fwd_any<float2>(v, dspec, s);
 }
 
 hipError_t launch_fft_inv32(const VlistArgs& v, const float2* espec, hipStream_t s) {
   if (v.ntrg <= 0) return hipSuccess;
-  if (v.np > kMaxNp) return hipErrorInvalidValue;
+  if (v.np > kMaxNp || v.nrhs < 1) return hipErrorInvalidValue;
   return launch_fft_inv_any<float2>(v, espec, s);
 }
 
-// ---------------------------------------------------------------------------
-// FP64 direct sum at sampled targets (dense3d_mpi.cpp:122-129): one workgroup
-// per target, lanes stride over the sources, LDS tree reduction.
 // ---------------------------------------------------------------------------
 // FP64 direct sum (Dense3d_MPI::evaluate, dense3d_mpi.cpp:75-141) at nt
 // check targets over all sources.  Workgroup (x, y) takes 256 targets (one per
@@ -742,28 +808,49 @@ __global__ void k_direct_sum(const double* __restrict__ part, int nchunk, int64_
 // ---------------------------------------------------------------------------
 #define KIFMM_USER_KERNELS(X) X(111) X(121) X(311) X(312) X(321) X(322) X(511) X(521)
 
-hipError_t launch_near(int kt, const NearArgs& a, const KParams<double>& pu, const KParams<double>& pm,
+hipError_t launch_near(int kt, const NearArgs& a0, const KParams<double>& pu, const KParams<double>& pm,
                        hipStream_t s) {
-  if (a.nchunks <= 0) return hipSuccess;
-  switch (kt) {
-#define KIFMM_CASE(K) case K: k_near<K><<<dim3(unsigned(a.nchunks)), dim3(kNearChunk), 0, s>>>(a, pu, pm); break;
-    KIFMM_USER_KERNELS(KIFMM_CASE)
+  if (a0.nchunks <= 0) return hipSuccess;
+  if (a0.nrhs < 1) return hipErrorInvalidValue;
+  for (int r0 = 0; r0 < a0.nrhs; r0 += kMaxNr) {
+    NearArgs a = a0;
+    a.r0 = r0;
+    a.nr = std::min(kMaxNr, a0.nrhs - r0);
+    const dim3 g(unsigned(a.nchunks)), b(kNearChunk);
+    switch (kt * 16 + rhs_group(a.nr)) {
+#define KIFMM_CASE_NR(K, R) case K * 16 + R: k_near<K, R><<<g, b, 0, s>>>(a, pu, pm); break;
+#define KIFMM_CASE(K) KIFMM_CASE_NR(K, 1) KIFMM_CASE_NR(K, 2) KIFMM_CASE_NR(K, 4) KIFMM_CASE_NR(K, 8)
+      KIFMM_USER_KERNELS(KIFMM_CASE)
 #undef KIFMM_CASE
-    default: return hipErrorInvalidValue;
+#undef KIFMM_CASE_NR
+      default: return hipErrorInvalidValue;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
   }
-  return hipGetLastError();
+  return hipSuccess;
 }
 
-hipError_t launch_surface(int kt, const SurfArgs& a, const KParams<double>& pu, hipStream_t s) {
-  if (a.ntasks <= 0) return hipSuccess;
-  if (a.n_sam > 2 * kSurfThreads) return hipErrorInvalidValue;
-  switch (kt) {
-#define KIFMM_CASE(K) case K: k_surface<K><<<dim3(unsigned(a.ntasks)), dim3(kSurfThreads), 0, s>>>(a, pu); break;
-    KIFMM_USER_KERNELS(KIFMM_CASE)
+hipError_t launch_surface(int kt, const SurfArgs& a0, const KParams<double>& pu, hipStream_t s) {
+  if (a0.ntasks <= 0) return hipSuccess;
+  if (a0.n_sam > 2 * kSurfThreads || a0.nrhs < 1) return hipErrorInvalidValue;
+  for (int r0 = 0; r0 < a0.nrhs; r0 += kMaxNr) {
+    SurfArgs a = a0;
+    a.r0 = r0;
+    a.nr = std::min(kMaxNr, a0.nrhs - r0);
+    const dim3 g(unsigned(a.ntasks)), b(kSurfThreads);
+    switch (kt * 16 + rhs_group(a.nr)) {
+#define KIFMM_CASE_NR(K, R) case K * 16 + R: k_surface<K, R><<<g, b, 0, s>>>(a, pu); break;
+#define KIFMM_CASE(K) KIFMM_CASE_NR(K, 1) KIFMM_CASE_NR(K, 2) KIFMM_CASE_NR(K, 4) KIFMM_CASE_NR(K, 8)
+      KIFMM_USER_KERNELS(KIFMM_CASE)
 #undef KIFMM_CASE
-    default: return hipErrorInvalidValue;
+#undef KIFMM_CASE_NR
+      default: return hipErrorInvalidValue;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
   }
-  return hipGetLastError();
+  return hipSuccess;
 }
 
 DirectPlan plan_direct(int64_t ns, int nt, int td, int64_t cap_doubles) {

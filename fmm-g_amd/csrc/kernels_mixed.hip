@@ -10,6 +10,8 @@
 //   translations and pseudo-inverse applies   FP64 GEMM with FMA
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device.hpp"
 #include "gemm.hpp"
 
@@ -18,40 +20,49 @@ namespace kifmm {
 static KParams<float> to_f32(const KParams<double>& p) { return KParams<float>{float(p.a), float(p.b), float(p.c)}; }
 
 // ---------------------------------------------------------------------------
-template <int KT>
+// den: NR right-hand sides of SD values; acc[r][t].  The kernel block of a pair
+// is formed once and applied to every right-hand side.
+template <int KT, int NR>
 __device__ inline void pair_f32(const KParams<float>& p, float dx, float dy, float dz, float nx, float ny,
-                                float nz, const float* den, float* acc) {
+                                float nz, const float* den, float (*acc)[trg_dof(KT)]) {
   constexpr int SD = src_dof(KT), TD = trg_dof(KT);
   if constexpr (KT == 111) {
     const float r2 = dx * dx + dy * dy + dz * dz;
     const float g = r2 < float(kMinDif * kMinDif) ? 0.0f : p.a * rsqrtf(r2);
-    acc[0] = fmaf(g, den[0], acc[0]);
+#pragma unroll
+    for (int r = 0; r < NR; r++) acc[r][0] = fmaf(g, den[r], acc[r][0]);
   } else {
     float blk[SD * TD];
     kblock<KT, float>(p, dx, dy, dz, nx, ny, nz, blk);
 #pragma unroll
-    for (int t = 0; t < TD; t++)
+    for (int r = 0; r < NR; r++)
 #pragma unroll
-      for (int s = 0; s < SD; s++) acc[t] = fmaf(blk[t * SD + s], den[s], acc[t]);
+      for (int t = 0; t < TD; t++)
+#pragma unroll
+        for (int s = 0; s < SD; s++) acc[r][t] = fmaf(blk[t * SD + s], den[r * SD + s], acc[r][t]);
   }
 }
 
 // Double-layer kernels: r and r.n from FP64 positions (r.n cancels for close
 // pairs on a smooth surface), the rest of the kernel in FP32.
-template <int KT>
+template <int KT, int NR>
 __device__ inline void pair_f32_dl(const KParams<float>& p, double dx, double dy, double dz, float nx, float ny,
-                                   float nz, const float* den, float* acc) {
+                                   float nz, const float* den, float (*acc)[trg_dof(KT)]) {
   constexpr int SD = src_dof(KT), TD = trg_dof(KT);
   const float rn = float(dx * double(nx) + dy * double(ny) + dz * double(nz));
   float blk[SD * TD];
   kblock_rn<KT, float>(p, float(dx), float(dy), float(dz), nx, ny, nz, rn, blk);
 #pragma unroll
-  for (int t = 0; t < TD; t++)
+  for (int r = 0; r < NR; r++)
 #pragma unroll
-    for (int s = 0; s < SD; s++) acc[t] = fmaf(blk[t * SD + s], den[s], acc[t]);
+    for (int t = 0; t < TD; t++)
+#pragma unroll
+      for (int s = 0; s < SD; s++) acc[r][t] = fmaf(blk[t * SD + s], den[r * SD + s], acc[r][t]);
 }
 
-template <int KT>
+// NR right-hand sides per launch (see NearArgs): the sources are staged once
+// with all their densities.
+template <int KT, int NR>
 __global__ __launch_bounds__(kNearChunk) void k_near_mx(NearArgs a, KParams<float> pu, KParams<float> pm) {
   constexpr int KM = mm_kernel(KT);
   constexpr int SD = src_dof(KT), TD = trg_dof(KT), sd = src_dof(KM);
@@ -60,7 +71,7 @@ __global__ __launch_bounds__(kNearChunk) void k_near_mx(NearArgs a, KParams<floa
   __shared__ float sp[kNearChunk * 3];
   __shared__ float sn[NOR ? kNearChunk * 3 : 1];
   __shared__ double sp64[NOR ? kNearChunk * 3 : 1];
-  __shared__ float sv[kNearChunk * MAXD];
+  __shared__ float sv[kNearChunk * MAXD * NR];
   __shared__ double red[kNearChunk * TD * 2];
   const NearChunk ch = a.chunks[blockIdx.x];
   const double4 cb = a.geom[ch.node];
@@ -76,16 +87,20 @@ __global__ __launch_bounds__(kNearChunk) void k_near_mx(NearArgs a, KParams<floa
     tx64 = p[0]; ty64 = p[1]; tz64 = p[2];
     tx = float(p[0] - cb.x); ty = float(p[1] - cb.y); tz = float(p[2] - cb.z);
   }
-  double near[TD], far[TD];
+  double near[NR][TD], far[NR][TD];
 #pragma unroll
-  for (int t = 0; t < TD; t++) near[t] = far[t] = 0.0;
+  for (int r = 0; r < NR; r++)
+#pragma unroll
+    for (int t = 0; t < TD; t++) near[r][t] = far[r][t] = 0.0;
 
   for (int it = ch.ibeg; it < ch.iend; it++) {
     const int code = a.items[it];
     const int node = code >> 2, kind = code & 3;
-    float acc[TD];
+    float acc[NR][TD];
 #pragma unroll
-    for (int t = 0; t < TD; t++) acc[t] = 0.0f;
+    for (int r = 0; r < NR; r++)
+#pragma unroll
+      for (int t = 0; t < TD; t++) acc[r][t] = 0.0f;
     if (kind == ITEM_EXACT) {
       const int b = a.src_beg[node], ns = a.src_num[node];
       for (int base = 0; base < ns; base += kNearChunk) {
@@ -98,28 +113,35 @@ __global__ __launch_bounds__(kNearChunk) void k_near_mx(NearArgs a, KParams<floa
           sp[3 * lane + 2] = float(a.spos[3 * g + 2] - cb.z);
           if constexpr (NOR)
             for (int d = 0; d < 3; d++) { sn[3 * lane + d] = float(a.snor[3 * g + d]); sp64[3 * lane + d] = a.spos[3 * g + d]; }
-          for (int s = 0; s < SD; s++) sv[lane * SD + s] = float(a.sden[g * SD + s]);
+#pragma unroll
+          for (int r = 0; r < NR; r++)
+            for (int s = 0; s < SD; s++)
+              sv[(lane * NR + r) * SD + s] = r < a.nr ? float(a.sden[(g * a.nrhs + a.r0 + r) * SD + s]) : 0.f;
         }
         __syncthreads();
         if (active) {
           for (int j = sub; j < m; j += S) {
             if constexpr (NOR) {
-              pair_f32_dl<KT>(pu, tx64 - sp64[3 * j], ty64 - sp64[3 * j + 1], tz64 - sp64[3 * j + 2], sn[3 * j],
-                              sn[3 * j + 1], sn[3 * j + 2], &sv[j * SD], acc);
+              pair_f32_dl<KT, NR>(pu, tx64 - sp64[3 * j], ty64 - sp64[3 * j + 1], tz64 - sp64[3 * j + 2], sn[3 * j],
+                                  sn[3 * j + 1], sn[3 * j + 2], &sv[j * NR * SD], acc);
             } else {
-              pair_f32<KT>(pu, tx - sp[3 * j], ty - sp[3 * j + 1], tz - sp[3 * j + 2], 0.f, 0.f, 0.f, &sv[j * SD], acc);
+              pair_f32<KT, NR>(pu, tx - sp[3 * j], ty - sp[3 * j + 1], tz - sp[3 * j + 2], 0.f, 0.f, 0.f,
+                               &sv[j * NR * SD], acc);
             }
           }
         }
       }
 #pragma unroll
-      for (int t = 0; t < TD; t++) near[t] += double(acc[t]);
+      for (int r = 0; r < NR; r++)
+#pragma unroll
+        for (int t = 0; t < TD; t++) near[r][t] += double(acc[r][t]);
     } else {
       const double4 g = a.geom[node];
       const bool ue = kind == ITEM_UE;
       const double* sam = ue ? a.sam_ue : a.sam_de;
       const int ns = ue ? a.n_ue : a.n_de;
       const double* den = ue ? a.UE + int64_t(node) * a.sz_ue : a.DE + int64_t(node) * a.sz_de;
+      const int64_t rstep = a.nstride * (ue ? a.sz_ue : a.sz_de);
       for (int base = 0; base < ns; base += kNearChunk) {
         const int m = min(kNearChunk, ns - base);
         __syncthreads();
@@ -128,41 +150,54 @@ __global__ __launch_bounds__(kNearChunk) void k_near_mx(NearArgs a, KParams<floa
           sp[3 * lane] = float((g.x - cb.x) + g.w * sam[3 * q]);
           sp[3 * lane + 1] = float((g.y - cb.y) + g.w * sam[3 * q + 1]);
           sp[3 * lane + 2] = float((g.z - cb.z) + g.w * sam[3 * q + 2]);
-          for (int s = 0; s < sd; s++) sv[lane * sd + s] = float(den[q * sd + s]);
+#pragma unroll
+          for (int r = 0; r < NR; r++)
+            for (int s = 0; s < sd; s++)
+              sv[(lane * NR + r) * sd + s] = r < a.nr ? float(den[(a.r0 + r) * rstep + q * sd + s]) : 0.f;
         }
         __syncthreads();
         if (active) {
           for (int j = sub; j < m; j += S) {
             const float x = sp[3 * j], y = sp[3 * j + 1], z = sp[3 * j + 2];
-            pair_f32<KM>(pm, tx - x, ty - y, tz - z, x, y, z, &sv[j * sd], acc);
+            pair_f32<KM, NR>(pm, tx - x, ty - y, tz - z, x, y, z, &sv[j * NR * sd], acc);
           }
         }
       }
       if (ue) {
 #pragma unroll
-        for (int t = 0; t < TD; t++) near[t] += double(acc[t]);
+        for (int r = 0; r < NR; r++)
+#pragma unroll
+          for (int t = 0; t < TD; t++) near[r][t] += double(acc[r][t]);
       } else {
 #pragma unroll
-        for (int t = 0; t < TD; t++) far[t] += double(acc[t]);
+        for (int r = 0; r < NR; r++)
+#pragma unroll
+          for (int t = 0; t < TD; t++) far[r][t] += double(acc[r][t]);
       }
     }
   }
-  __syncthreads();
-  if (active && sub > 0)
-    for (int t = 0; t < TD; t++) {
-      red[((lane - ch.tnum) * TD + t) * 2] = near[t];
-      red[((lane - ch.tnum) * TD + t) * 2 + 1] = far[t];
-    }
-  __syncthreads();
-  if (lane < ch.tnum) {
-    for (int u = 1; u < S; u++)
-      for (int t = 0; t < TD; t++) {
-        near[t] += red[(((u - 1) * ch.tnum + lane) * TD + t) * 2];
-        far[t] += red[(((u - 1) * ch.tnum + lane) * TD + t) * 2 + 1];
-      }
-    double* o = a.tval + int64_t(ch.tbeg + lane) * TD;
 #pragma unroll
-    for (int t = 0; t < TD; t++) o[t] = near[t] + far[t];
+  for (int r = 0; r < NR; r++) {
+    __syncthreads();
+    if (active && sub > 0)
+      for (int t = 0; t < TD; t++) {
+        red[((lane - ch.tnum) * TD + t) * 2] = near[r][t];
+        red[((lane - ch.tnum) * TD + t) * 2 + 1] = far[r][t];
+      }
+    __syncthreads();
+    if (lane < ch.tnum && r < a.nr) {
+      double nv[TD], fv[TD];
+#pragma unroll
+      for (int t = 0; t < TD; t++) { nv[t] = near[r][t]; fv[t] = far[r][t]; }
+      for (int u = 1; u < S; u++)
+        for (int t = 0; t < TD; t++) {
+          nv[t] += red[(((u - 1) * ch.tnum + lane) * TD + t) * 2];
+          fv[t] += red[(((u - 1) * ch.tnum + lane) * TD + t) * 2 + 1];
+        }
+      double* o = a.tval + int64_t(a.r0 + r) * a.tstride + int64_t(ch.tbeg + lane) * TD;
+#pragma unroll
+      for (int t = 0; t < TD; t++) o[t] = nv[t] + fv[t];
+    }
   }
 }
 
@@ -217,9 +252,30 @@ __device__ __forceinline__ void lap_sweep_nc(const float4* s4, int j0, int j1, f
   }
 }
 
+// Several right-hand sides (NR > 1): the positions sit in s4 and the NR
+// weights of a source side by side in sw; one rsqrt per pair serves all of them.
+template <int NR, bool CUT>
+__device__ __forceinline__ void lap_sweep_nr(const float4* s4, const float* sw, int j0, int j1, float tx, float ty,
+                                             float tz, float (&acc)[NR]) {
+  const float eps2 = float(kMinDif * kMinDif);
+  for (int j = j0; j < j1; j += 2) {
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const float4 s0 = s4[j + u];
+      const float dx = tx - s0.x, dy = ty - s0.y, dz = tz - s0.z;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      const float g = CUT ? (r2 < eps2 ? 0.f : rsqrtf(r2)) : __builtin_amdgcn_rsqf(r2);
+#pragma unroll
+      for (int r = 0; r < NR; r++) acc[r] = fmaf(g, sw[(j + u) * NR + r], acc[r]);
+    }
+  }
+}
+
 constexpr int kItemWin = 128;
+template <int NR>
 __global__ __launch_bounds__(kNearChunk) void k_near_lap_mx(NearArgs a, float pa_u, float pa_m) {
   __shared__ float4 s4[kNearChunk];
+  __shared__ float sw[NR > 1 ? kNearChunk * NR : 1];
   __shared__ int pre[kItemWin + 1];
   __shared__ int ibeg_[kItemWin];
   __shared__ unsigned char ikind_[kItemWin];
@@ -239,7 +295,9 @@ __global__ __launch_bounds__(kNearChunk) void k_near_lap_mx(NearArgs a, float pa
     tx = float(p[0] - cb.x); ty = float(p[1] - cb.y); tz = float(p[2] - cb.z);
   }
   const int idiv = ch.iend > ch.ibeg && (a.items[ch.iend - 1] & 3) == ITEM_DE ? ch.iend - 1 : ch.iend;
-  double part[2] = {0.0, 0.0};
+  double part[2][NR];
+#pragma unroll
+  for (int r = 0; r < NR; r++) part[0][r] = part[1][r] = 0.0;
   for (int pass = 0; pass < 2; pass++) {
     const int i0 = pass ? idiv : ch.ibeg, i1 = pass ? ch.iend : idiv;
     for (int w0 = i0; w0 < i1; w0 += kItemWin) {
@@ -274,6 +332,9 @@ __global__ __launch_bounds__(kNearChunk) void k_near_lap_mx(NearArgs a, float pa
         const bool cut = !ch.unchk || (own0 < base + m && own1 > base);
         __syncthreads();
         float4 v = make_float4(1e18f, 0.f, 0.f, 0.f);
+        float wv[NR];
+#pragma unroll
+        for (int r = 0; r < NR; r++) wv[r] = 0.f;
         if (lane < m) {
           const int p = base + lane;
           int lo = 0, hi = nw - 1;
@@ -286,15 +347,28 @@ __global__ __launch_bounds__(kNearChunk) void k_near_lap_mx(NearArgs a, float pa
           if (kind == ITEM_EXACT) {
             const int64_t q = int64_t(ibeg_[lo] + off);
             v = make_float4(float(a.spos[3 * q] - cb.x), float(a.spos[3 * q + 1] - cb.y),
-                            float(a.spos[3 * q + 2] - cb.z), float(a.sden[q]) * pa_u);
+                            float(a.spos[3 * q + 2] - cb.z), 0.f);
+#pragma unroll
+            for (int r = 0; r < NR; r++)
+              if (r < a.nr) wv[r] = float(a.sden[q * a.nrhs + a.r0 + r]) * pa_u;
           } else {
             const int node = ibeg_[lo];
             const double4 g = a.geom[node];
             const double* sam = kind == ITEM_UE ? a.sam_ue : a.sam_de;
             const double* den = kind == ITEM_UE ? a.UE + int64_t(node) * a.sz_ue : a.DE + int64_t(node) * a.sz_de;
+            const int64_t rstep = a.nstride * (kind == ITEM_UE ? a.sz_ue : a.sz_de);
             v = make_float4(float((g.x - cb.x) + g.w * sam[3 * off]), float((g.y - cb.y) + g.w * sam[3 * off + 1]),
-                            float((g.z - cb.z) + g.w * sam[3 * off + 2]), float(den[off]) * pa_m);
+                            float((g.z - cb.z) + g.w * sam[3 * off + 2]), 0.f);
+#pragma unroll
+            for (int r = 0; r < NR; r++)
+              if (r < a.nr) wv[r] = float(den[(a.r0 + r) * rstep + off]) * pa_m;
           }
+        }
+        if constexpr (NR == 1) {
+          v.w = wv[0];
+        } else {
+#pragma unroll
+          for (int r = 0; r < NR; r++) sw[lane * NR + r] = wv[r];
         }
         s4[lane] = v;
         __syncthreads();
@@ -302,41 +376,54 @@ __global__ __launch_bounds__(kNearChunk) void k_near_lap_mx(NearArgs a, float pa
           const int m4 = (m + 3) & ~3;
           const int L = ((m4 + S - 1) / S + 3) & ~3;
           const int j0 = sub * L, j1 = min(j0 + L, m4);
-          float acc0 = 0.f, acc1 = 0.f;
-          if (cut) lap_sweep(s4, j0, j1, tx, ty, tz, acc0, acc1);
-          else lap_sweep_nc(s4, j0, j1, tx, ty, tz, acc0, acc1);
-          part[pass] += double(acc0 + acc1);
+          if constexpr (NR == 1) {
+            float acc0 = 0.f, acc1 = 0.f;
+            if (cut) lap_sweep(s4, j0, j1, tx, ty, tz, acc0, acc1);
+            else lap_sweep_nc(s4, j0, j1, tx, ty, tz, acc0, acc1);
+            part[pass][0] += double(acc0 + acc1);
+          } else {
+            float acc[NR];
+#pragma unroll
+            for (int r = 0; r < NR; r++) acc[r] = 0.f;
+            if (cut) lap_sweep_nr<NR, true>(s4, sw, j0, j1, tx, ty, tz, acc);
+            else lap_sweep_nr<NR, false>(s4, sw, j0, j1, tx, ty, tz, acc);
+#pragma unroll
+            for (int r = 0; r < NR; r++) part[pass][r] += double(acc[r]);
+          }
         }
       }
     }
   }
-  __syncthreads();
-  if (active && sub > 0) {
-    red[(lane - tn) * 2] = part[0];
-    red[(lane - tn) * 2 + 1] = part[1];
-  }
-  __syncthreads();
-  if (lane < ch.tnum) {
-    double near = part[0], far = part[1];
-    for (int u = 1; u < S; u++) {
-      near += red[((u - 1) * tn + lane) * 2];
-      far += red[((u - 1) * tn + lane) * 2 + 1];
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    __syncthreads();
+    if (active && sub > 0) {
+      red[(lane - tn) * 2] = part[0][r];
+      red[(lane - tn) * 2 + 1] = part[1][r];
     }
-    const int64_t o = ch.tbeg + lane;
-    a.tval[a.tperm ? int64_t(a.tperm[o]) : o] = near + far;
+    __syncthreads();
+    if (lane < ch.tnum && r < a.nr) {
+      double near = part[0][r], far = part[1][r];
+      for (int u = 1; u < S; u++) {
+        near += red[((u - 1) * tn + lane) * 2];
+        far += red[((u - 1) * tn + lane) * 2 + 1];
+      }
+      const int64_t o = ch.tbeg + lane;
+      a.tval[int64_t(a.r0 + r) * a.tstride + (a.tperm ? int64_t(a.tperm[o]) : o)] = near + far;
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
 constexpr int kSurfMx = 256;
 
-template <int KT>
+template <int KT, int NR>
 __global__ __launch_bounds__(kSurfMx) void k_surface_mx(SurfArgs a, KParams<float> pu) {
   constexpr int SD = src_dof(KT), TD = trg_dof(KT);
   constexpr bool NOR = needs_normals_ct(KT);
   __shared__ float sp[kSurfMx * 3];
   __shared__ float sn[NOR ? kSurfMx * 3 : 1];
-  __shared__ float sv[kSurfMx * SD];
+  __shared__ float sv[kSurfMx * SD * NR];
   int node, ib, ie, sb1 = 0, sn1 = 0;
   if (a.leaf) {
     const int4 L = a.leaf[blockIdx.x];
@@ -348,14 +435,16 @@ __global__ __launch_bounds__(kSurfMx) void k_surface_mx(SurfArgs a, KParams<floa
   const double4 g = a.geom[node];
   const int lane = threadIdx.x;
   float tp[2][3];
-  double acc[2][TD];
+  double acc[2][NR][TD];
   bool act[2];
   for (int u = 0; u < 2; u++) {
     const int q = lane + u * kSurfMx;
     act[u] = q < a.n_sam;
     const int qq = act[u] ? q : 0;
     for (int d = 0; d < 3; d++) tp[u][d] = float(g.w * a.sam[3 * qq + d]);
-    for (int t = 0; t < TD; t++) acc[u][t] = 0.0;
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+      for (int t = 0; t < TD; t++) acc[u][r][t] = 0.0;
   }
   for (int it = ib; it < ie; it++) {
     int b = sb1, ns = sn1;
@@ -373,14 +462,20 @@ __global__ __launch_bounds__(kSurfMx) void k_surface_mx(SurfArgs a, KParams<floa
         sp[3 * lane + 2] = float(a.spos[3 * gg + 2] - g.z);
         if constexpr (NOR) for (int d = 0; d < 3; d++) sn[3 * lane + d] = float(a.snor[3 * gg + d]);
         // Laplace: the 1/(4 pi) factor goes with the density
-        for (int s = 0; s < SD; s++) sv[lane * SD + s] = float(a.sden[gg * SD + s]) * (KT == 111 ? pu.a : 1.f);
+#pragma unroll
+        for (int r = 0; r < NR; r++)
+          for (int s = 0; s < SD; s++)
+            sv[(lane * NR + r) * SD + s] =
+                r < a.nr ? float(a.sden[(gg * a.nrhs + a.r0 + r) * SD + s]) * (KT == 111 ? pu.a : 1.f) : 0.f;
       }
       __syncthreads();
 #pragma unroll
       for (int u = 0; u < 2; u++) {
         if (!act[u]) continue;
-        float part[TD];
-        for (int t = 0; t < TD; t++) part[t] = 0.0f;
+        float part[NR][TD];
+#pragma unroll
+        for (int r = 0; r < NR; r++)
+          for (int t = 0; t < TD; t++) part[r][t] = 0.0f;
         if constexpr (KT == 111) {
           // check surfaces never meet their sources: the upward check surface
           // lies outside the leaf, and an X-list leaf is not adjacent to the
@@ -388,29 +483,37 @@ __global__ __launch_bounds__(kSurfMx) void k_surface_mx(SurfArgs a, KParams<floa
           // pair is within the 1e-8 cut and r^2 is a normal float
           for (int j = 0; j < m; j++) {
             const float dx = tp[u][0] - sp[3 * j], dy = tp[u][1] - sp[3 * j + 1], dz = tp[u][2] - sp[3 * j + 2];
-            part[0] = fmaf(__builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz), sv[j], part[0]);
+            const float gk = __builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz);
+#pragma unroll
+            for (int r = 0; r < NR; r++) part[r][0] = fmaf(gk, sv[j * NR + r], part[r][0]);
           }
         } else {
           for (int j = 0; j < m; j++) {
             float nx = 0, ny = 0, nz = 0;
             if constexpr (NOR) { nx = sn[3 * j]; ny = sn[3 * j + 1]; nz = sn[3 * j + 2]; }
-            pair_f32<KT>(pu, tp[u][0] - sp[3 * j], tp[u][1] - sp[3 * j + 1], tp[u][2] - sp[3 * j + 2], nx, ny, nz,
-                         &sv[j * SD], part);
+            pair_f32<KT, NR>(pu, tp[u][0] - sp[3 * j], tp[u][1] - sp[3 * j + 1], tp[u][2] - sp[3 * j + 2], nx, ny, nz,
+                             &sv[j * NR * SD], part);
           }
         }
-        for (int t = 0; t < TD; t++) acc[u][t] += double(part[t]);
+#pragma unroll
+        for (int r = 0; r < NR; r++)
+          for (int t = 0; t < TD; t++) acc[u][r][t] += double(part[r][t]);
       }
     }
   }
   // one-leaf tasks (S2M) are the only writers of their node's check surface:
   // store; other tasks (S2L) add to what the V list already put there
-  double* o = a.out + int64_t(node) * a.sz_out;
   for (int u = 0; u < 2; u++) {
     if (!act[u]) continue;
     const int q = lane + u * kSurfMx;
-    for (int t = 0; t < TD; t++) {
-      if (a.leaf) o[q * TD + t] = acc[u][t];
-      else o[q * TD + t] += acc[u][t];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      if (r >= a.nr) continue;
+      double* o = a.out + (int64_t(a.r0 + r) * a.nstride + node) * a.sz_out;
+      for (int t = 0; t < TD; t++) {
+        if (a.leaf) o[q * TD + t] = acc[u][r][t];
+        else o[q * TD + t] += acc[u][r][t];
+      }
     }
   }
 }
@@ -422,7 +525,8 @@ template <int SD, int TD>
 __global__ __launch_bounds__(64) void k_hadamard_mx(VlistArgs v, const float2* __restrict__ dspec,
                                                    const float2* __restrict__ m2l, float2* __restrict__ espec,
                                                    float nrm) {
-  const int g = blockIdx.x;
+  const int g = blockIdx.x / v.nrhs, rh = blockIdx.x % v.nrhs;  // virtual group g * nrhs + rh
+  const int soff = rh * v.nsrc, toff = rh * v.ntrg;
   const int f = blockIdx.y * 64 + threadIdx.x;
   const int H = v.H;
   const bool ok = f < H;
@@ -440,7 +544,7 @@ __global__ __launch_bounds__(64) void k_hadamard_mx(VlistArgs v, const float2* _
     if (s < 0) continue;
     float2 d[SD];
 #pragma unroll
-    for (int j = 0; j < SD; j++) d[j] = dspec[(int64_t(s) * SD + j) * H + ff];
+    for (int j = 0; j < SD; j++) d[j] = dspec[(int64_t(s + soff) * SD + j) * H + ff];
     const int rs = (word >> 8) & 0x3ff, tm = word >> 20;
 #pragma unroll
     for (int tau = 0; tau < 8; tau++) {
@@ -464,7 +568,7 @@ __global__ __launch_bounds__(64) void k_hadamard_mx(VlistArgs v, const float2* _
     if (t < 0) continue;
 #pragma unroll
     for (int i = 0; i < TD; i++)
-      espec[(int64_t(t) * TD + i) * H + f] = make_float2(acc[tau][i].x * nrm, acc[tau][i].y * nrm);
+      espec[(int64_t(t + toff) * TD + i) * H + f] = make_float2(acc[tau][i].x * nrm, acc[tau][i].y * nrm);
   }
 }
 
@@ -514,12 +618,15 @@ __global__ __launch_bounds__(64) void k_m2l_tile(VlistArgs v, const float2* __re
   const int f = blockIdx.y * 64 + threadIdx.x;
   const bool fok = f < H;
   const int fs = fok ? f : H - 1;
+  // virtual groups g * nrhs + r: right-hand side r of sibling group g reads
+  // source spectra r * nsrc + slot and writes target spectra r * ntrg + slot,
+  // and consecutive virtual groups share the tile's translation spectra
   const int g0 = tile * G;
-  const int ng = min(G, v.ngrp - g0);
+  const int ng = min(G, v.ngrp * v.nrhs - g0);
   const unsigned H8 = unsigned(H) * 8u, fs8 = unsigned(fs) * 8u;
   // (unused when !BUF)
-  const __amdgpu_buffer_rsrc_t rs_x =
-      __builtin_amdgcn_make_buffer_rsrc((void*)dspec, (short)0, int(unsigned(v.nsrc) * H8), 0x00020000);
+  const __amdgpu_buffer_rsrc_t rs_x = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)dspec, (short)0, int(unsigned(v.nsrc) * unsigned(v.nrhs) * H8), 0x00020000);
   const __amdgpu_buffer_rsrc_t rs_t =
       __builtin_amdgcn_make_buffer_rsrc((void*)m2l, (short)0, int(343u * H8), 0x00020000);
   auto ldx = [&](int row) -> float2 {
@@ -539,7 +646,15 @@ __global__ __launch_bounds__(64) void k_m2l_tile(VlistArgs v, const float2* __re
     }
   };
   __shared__ int shal[G * 216];
-  for (int i = threadIdx.x; i < G * 216; i += 64) shal[i] = i < ng * 216 ? v.halo[int64_t(g0) * 216 + i] : -1;
+  for (int i = threadIdx.x; i < G * 216; i += 64) {
+    int h = -1;
+    if (i < ng * 216) {
+      const int vg = g0 + i / 216;
+      h = v.halo[int64_t(vg / v.nrhs) * 216 + i % 216];
+      if (h >= 0) h += (vg % v.nrhs) * v.nsrc;
+    }
+    shal[i] = h;
+  }
   __syncthreads();
   float k1[G][8], k2[G][8], k3[G][8];
 #pragma unroll
@@ -607,7 +722,9 @@ __global__ __launch_bounds__(64) void k_m2l_tile(VlistArgs v, const float2* __re
     if (gi >= ng) break;
 #pragma unroll
     for (int t = 0; t < 8; t++) {
-      const int tg = v.tgt[(g0 + gi) * 8 + t];
+      const int vg = g0 + gi;
+      int tg = v.tgt[(vg / v.nrhs) * 8 + t];
+      if (tg >= 0) tg += (vg % v.nrhs) * v.ntrg;
       if (tg >= 0)
         espec[int64_t(tg) * H + f] =
             make_float2((k1[gi][t] - k2[gi][t]) * nrm, (k3[gi][t] - k1[gi][t] - k2[gi][t]) * nrm);
@@ -618,46 +735,75 @@ __global__ __launch_bounds__(64) void k_m2l_tile(VlistArgs v, const float2* __re
 // ---------------------------------------------------------------------------
 #define KIFMM_USER_KERNELS_MX(X) X(111) X(121) X(311) X(312) X(321) X(322) X(511) X(521)
 
-hipError_t launch_near_mixed(int kt, const NearArgs& a, const KParams<double>& pu, const KParams<double>& pm,
+hipError_t launch_near_mixed(int kt, const NearArgs& a0, const KParams<double>& pu, const KParams<double>& pm,
                              hipStream_t s) {
-  if (a.nchunks <= 0) return hipSuccess;
+  if (a0.nchunks <= 0) return hipSuccess;
+  if (a0.nrhs < 1) return hipErrorInvalidValue;
   const KParams<float> fu = to_f32(pu), fm = to_f32(pm);
-  if (kt == 111) {
-    k_near_lap_mx<<<dim3(unsigned(a.nchunks)), dim3(kNearChunk), 0, s>>>(a, fu.a, fm.a);
-    return hipGetLastError();
-  }
-  switch (kt) {
-#define KIFMM_CASE(K) case K: k_near_mx<K><<<dim3(unsigned(a.nchunks)), dim3(kNearChunk), 0, s>>>(a, fu, fm); break;
-    KIFMM_USER_KERNELS_MX(KIFMM_CASE)
+  const dim3 g(unsigned(a0.nchunks)), b(kNearChunk);
+  for (int r0 = 0; r0 < a0.nrhs; r0 += kMaxNr) {
+    NearArgs a = a0;
+    a.r0 = r0;
+    a.nr = std::min(kMaxNr, a0.nrhs - r0);
+    const int nr = rhs_group(a.nr);
+    if (kt == 111) {
+      switch (nr) {
+        case 1: k_near_lap_mx<1><<<g, b, 0, s>>>(a, fu.a, fm.a); break;
+        case 2: k_near_lap_mx<2><<<g, b, 0, s>>>(a, fu.a, fm.a); break;
+        case 4: k_near_lap_mx<4><<<g, b, 0, s>>>(a, fu.a, fm.a); break;
+        default: k_near_lap_mx<8><<<g, b, 0, s>>>(a, fu.a, fm.a); break;
+      }
+    } else {
+      switch (kt * 16 + nr) {
+#define KIFMM_CASE_NR(K, R) case K * 16 + R: k_near_mx<K, R><<<g, b, 0, s>>>(a, fu, fm); break;
+#define KIFMM_CASE(K) KIFMM_CASE_NR(K, 1) KIFMM_CASE_NR(K, 2) KIFMM_CASE_NR(K, 4) KIFMM_CASE_NR(K, 8)
+        KIFMM_USER_KERNELS_MX(KIFMM_CASE)
 #undef KIFMM_CASE
-    default: return hipErrorInvalidValue;
+#undef KIFMM_CASE_NR
+        default: return hipErrorInvalidValue;
+      }
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
   }
-  return hipGetLastError();
+  return hipSuccess;
 }
 
-hipError_t launch_surface_mixed(int kt, const SurfArgs& a, const KParams<double>& pu, hipStream_t s) {
-  if (a.ntasks <= 0) return hipSuccess;
-  if (a.n_sam > 2 * kSurfMx) return hipErrorInvalidValue;
+hipError_t launch_surface_mixed(int kt, const SurfArgs& a0, const KParams<double>& pu, hipStream_t s) {
+  if (a0.ntasks <= 0) return hipSuccess;
+  if (a0.n_sam > 2 * kSurfMx || a0.nrhs < 1) return hipErrorInvalidValue;
   const KParams<float> fu = to_f32(pu);
-  switch (kt) {
-#define KIFMM_CASE(K) case K: k_surface_mx<K><<<dim3(unsigned(a.ntasks)), dim3(kSurfMx), 0, s>>>(a, fu); break;
-    KIFMM_USER_KERNELS_MX(KIFMM_CASE)
+  const dim3 g(unsigned(a0.ntasks)), b(kSurfMx);
+  for (int r0 = 0; r0 < a0.nrhs; r0 += kMaxNr) {
+    SurfArgs a = a0;
+    a.r0 = r0;
+    a.nr = std::min(kMaxNr, a0.nrhs - r0);
+    switch (kt * 16 + rhs_group(a.nr)) {
+#define KIFMM_CASE_NR(K, R) case K * 16 + R: k_surface_mx<K, R><<<g, b, 0, s>>>(a, fu); break;
+#define KIFMM_CASE(K) KIFMM_CASE_NR(K, 1) KIFMM_CASE_NR(K, 2) KIFMM_CASE_NR(K, 4) KIFMM_CASE_NR(K, 8)
+      KIFMM_USER_KERNELS_MX(KIFMM_CASE)
 #undef KIFMM_CASE
-    default: return hipErrorInvalidValue;
+#undef KIFMM_CASE_NR
+      default: return hipErrorInvalidValue;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
   }
-  return hipGetLastError();
+  return hipSuccess;
 }
 
 hipError_t launch_hadamard_mixed(const VlistArgs& v, const float2* dspec, const float2* m2l, float2* espec,
                                  hipStream_t s) {
   if (v.ngrp <= 0) return hipSuccess;
+  if (v.nrhs < 1) return hipErrorInvalidValue;
   const float nrm = float(1.0 / double(v.n * v.n * v.n));
-  dim3 grid(unsigned(v.ngrp), unsigned((v.H + 63) / 64));
+  const int64_t nvg = int64_t(v.ngrp) * v.nrhs;  // virtual groups
+  dim3 grid(unsigned(nvg), unsigned((v.H + 63) / 64));
   if (v.sd == 1 && v.td == 1) {
-    const int ntile = (v.ngrp + kTileG - 1) / kTileG;
+    const int ntile = int((nvg + kTileG - 1) / kTileG);
     const int per_xcd = (ntile + 7) / 8;
     const dim3 grid2(unsigned(8 * per_xcd), unsigned((v.H + 63) / 64));
-    if (uint64_t(v.nsrc) * v.H * 8 <= 0xFFFFFF00ull)
+    if (uint64_t(v.nsrc) * v.nrhs * v.H * 8 <= 0xFFFFFF00ull)
       k_m2l_tile<true><<<grid2, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm, ntile, per_xcd);
     else
       k_m2l_tile<false><<<grid2, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm, ntile, per_xcd);

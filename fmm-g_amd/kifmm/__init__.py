@@ -238,6 +238,13 @@ class FMM3d:
         self._chk(lib().kifmm_evaluate_multi(self._h, _pd(d), _pd(out), d.shape[0]))
         return out
 
+    def evaluate_multi_device(self, d_src_den, d_trg_val, nrhs, stream=None):
+        """nrhs right-hand sides in one pass on device tensors (nrhs vectors one
+        after another)."""
+        self._chk(lib().kifmm_evaluate_multi_device(
+            self._h, _dev(d_src_den, nrhs * self.n_src * self.src_dof, "densities"),
+            _dev(d_trg_val, nrhs * self.n_trg * self.trg_dof, "values"), int(nrhs), _stream(stream)))
+
     def evaluate_adjoint(self, trg_in):
         """Transposed pass A^T w: trgDOF per target in, srcDOF per source out."""
         w = _f64(trg_in, self.n_trg * self.trg_dof)

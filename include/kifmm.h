@@ -90,9 +90,14 @@ int kifmm_evaluate(kifmm_handle h, const double *src_den, double *trg_val);
 int kifmm_evaluate_device(kifmm_handle h, const double *d_src_den, double *d_trg_val, void *stream);
 
 /* nrhs right-hand sides in one call (the nrhs argument proposed for
- * KnlMat3d_MPI::evaluate, knlmat3d_mpi.hpp:62): src_den holds nrhs density
- * vectors one after another (nrhs x n_src x srcDOF), trg_val receives nrhs value
- * vectors (nrhs x n_trg x trgDOF), overwritten.  Host pointers, synchronous. */
+ * KnlMat3d_MPI::evaluate, knlmat3d_mpi.hpp:62, which takes one Vec per call):
+ * src_den holds nrhs density vectors one after another (nrhs x n_src x
+ * srcDOF), trg_val receives nrhs value vectors (nrhs x n_trg x trgDOF),
+ * overwritten.  One pass serves all right-hand sides: each phase is one
+ * launch over every right-hand side (groups of 8 for the point kernels), and
+ * each right-hand side's values equal those of a one-RHS evaluate to
+ * round-off (bitwise in FP64).  The handle's work buffers grow to nrhs blocks
+ * on the first call with a larger nrhs.  Host pointers, synchronous. */
 int kifmm_evaluate_multi(kifmm_handle h, const double *src_den, double *trg_val, int nrhs);
 
 /* Same with device pointers, enqueued on `stream`. */
