@@ -697,6 +697,15 @@ void split_k(GemmArgs& g) {
   if (g.ksplit <= 1) { g.ksplit = 0; g.kchunk = 0; }
 }
 
+void split_k_rhs(GemmArgs& g, int nrhs) {
+  GemmArgs g1 = g;
+  g1.ncol = g.ncol / nrhs;
+  g1.max_zcol = g.max_zcol / nrhs;
+  split_k(g1);
+  g.ksplit = g1.ksplit;
+  g.kchunk = g1.kchunk;
+}
+
 int FMM3d::evaluate(const double* d_src_den, double* d_trg_val, hipStream_t s) {
   if (!setup_done_) return fail(-3, "evaluate called before setup");
   if (!use_graph_ || !s || timing_) return evaluate_impl(d_src_den, d_trg_val, s, false);
@@ -831,7 +840,7 @@ int FMM3d::prepare_rhs_steps(int nrhs) {
         }
         g.ncol = st.ncol; g.max_zcol = st.ncol;
         if (st.zoff_off >= 0) { g.zoff = &st.ncol; g.nz = 8; g.max_zcol = st.max_zcol; }
-        split_k(g);
+        split_k_rhs(g, nrhs);
         if (g.ksplit > 1) need = std::max(need, size_t(g.ksplit) * size_t(g.ncol) * size_t(g.M));
       }
     if ((rc = reserve_splitk(need))) return rc;
@@ -926,7 +935,7 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
       // whole 16-deep slabs; the chunks' partial products go to the workspace
       // and are summed in chunk order into the output
       if (mixed) {
-        split_k(g);
+        split_k_rhs(g, R);
         g.work = splitk_for(g);
         if (g.ksplit > 1 && !g.work) return fail(-6, "split-K workspace smaller than the launch needs");
       }

@@ -56,6 +56,9 @@ struct VGroups {
 
 // Split K of a mixed-path level GEMM launch that would have too few workgroups.
 void split_k(GemmArgs& g);
+// The split of a launch over nrhs right-hand sides is the one-RHS launch's
+// (same K chunks per column), so every column sums in the one-RHS order.
+void split_k_rhs(GemmArgs& g, int nrhs);
 
 class FMM3d {
  public:

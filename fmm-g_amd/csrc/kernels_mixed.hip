@@ -253,20 +253,28 @@ __device__ __forceinline__ void lap_sweep_nc(const float4* s4, int j0, int j1, f
 }
 
 // Several right-hand sides (NR > 1): the positions sit in s4 and the NR
-// weights of a source side by side in sw; one rsqrt per pair serves all of them.
+// weights of a source side by side in sw; one rsqrt per pair serves all of
+// them.  Each right-hand side accumulates in the order of lap_sweep (even and
+// odd sources in two sums), so its values are those of a one-RHS launch.
 template <int NR, bool CUT>
 __device__ __forceinline__ void lap_sweep_nr(const float4* s4, const float* sw, int j0, int j1, float tx, float ty,
-                                             float tz, float (&acc)[NR]) {
+                                             float tz, float (&acc0)[NR], float (&acc1)[NR]) {
   const float eps2 = float(kMinDif * kMinDif);
-  for (int j = j0; j < j1; j += 2) {
+  for (int j = j0; j < j1; j += 4) {
 #pragma unroll
-    for (int u = 0; u < 2; u++) {
-      const float4 s0 = s4[j + u];
-      const float dx = tx - s0.x, dy = ty - s0.y, dz = tz - s0.z;
-      const float r2 = dx * dx + dy * dy + dz * dz;
-      const float g = CUT ? (r2 < eps2 ? 0.f : rsqrtf(r2)) : __builtin_amdgcn_rsqf(r2);
+    for (int u = 0; u < 4; u += 2) {
+      const float4 s0 = s4[j + u], s1 = s4[j + u + 1];
+      const float dx0 = tx - s0.x, dy0 = ty - s0.y, dz0 = tz - s0.z;
+      const float dx1 = tx - s1.x, dy1 = ty - s1.y, dz1 = tz - s1.z;
+      const float r0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
+      const float r1 = dx1 * dx1 + dy1 * dy1 + dz1 * dz1;
+      const float g0 = CUT ? (r0 < eps2 ? 0.f : rsqrtf(r0)) : __builtin_amdgcn_rsqf(r0);
+      const float g1 = CUT ? (r1 < eps2 ? 0.f : rsqrtf(r1)) : __builtin_amdgcn_rsqf(r1);
 #pragma unroll
-      for (int r = 0; r < NR; r++) acc[r] = fmaf(g, sw[(j + u) * NR + r], acc[r]);
+      for (int r = 0; r < NR; r++) {
+        acc0[r] = fmaf(g0, sw[(j + u) * NR + r], acc0[r]);
+        acc1[r] = fmaf(g1, sw[(j + u + 1) * NR + r], acc1[r]);
+      }
     }
   }
 }
@@ -382,13 +390,13 @@ __global__ __launch_bounds__(kNearChunk) void k_near_lap_mx(NearArgs a, float pa
             else lap_sweep_nc(s4, j0, j1, tx, ty, tz, acc0, acc1);
             part[pass][0] += double(acc0 + acc1);
           } else {
-            float acc[NR];
+            float acc0[NR], acc1[NR];
 #pragma unroll
-            for (int r = 0; r < NR; r++) acc[r] = 0.f;
-            if (cut) lap_sweep_nr<NR, true>(s4, sw, j0, j1, tx, ty, tz, acc);
-            else lap_sweep_nr<NR, false>(s4, sw, j0, j1, tx, ty, tz, acc);
+            for (int r = 0; r < NR; r++) acc0[r] = acc1[r] = 0.f;
+            if (cut) lap_sweep_nr<NR, true>(s4, sw, j0, j1, tx, ty, tz, acc0, acc1);
+            else lap_sweep_nr<NR, false>(s4, sw, j0, j1, tx, ty, tz, acc0, acc1);
 #pragma unroll
-            for (int r = 0; r < NR; r++) part[pass][r] += double(acc[r]);
+            for (int r = 0; r < NR; r++) part[pass][r] += double(acc0[r] + acc1[r]);
           }
         }
       }

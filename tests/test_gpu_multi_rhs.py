@@ -3,9 +3,9 @@
 Each phase runs one launch over all right-hand sides (point kernels in groups
 of up to 8 with every source staged once, GEMMs over nrhs times the columns, V
 list over nrhs times the boxes).  Every right-hand side's arithmetic is the
-one-RHS arithmetic, so in FP64 the values equal one-RHS evaluates bit for bit;
-in mixed precision the level GEMMs may split their K loop differently with
-more columns (FP64 partial sums in another order), so the bar there is 1e-9.
+one-RHS arithmetic in the same order (the FP32 near-field sums pair their
+sources as the one-RHS sweep does, the GEMMs keep the one-RHS K split), so in
+both precisions the values equal one-RHS evaluates bit for bit.
 """
 import numpy as np
 import pytest
@@ -37,16 +37,12 @@ def test_multi_equals_single(kt, coefs, np_, nrhs, precision):
     den = rng.standard_normal((nrhs, n * f.src_dof))
     multi = f.evaluate_multi(den)
     single = one_by_one(f, den)
-    if precision == "fp64":
-        assert np.array_equal(multi, single)
-    else:
-        r = np.linalg.norm(multi - single) / np.linalg.norm(single)
-        print("relative difference %.3e" % r, flush=True)
-        assert r < 1e-9
+    r = np.linalg.norm(multi - single) / np.linalg.norm(single)
+    print("relative difference %.3e" % r, flush=True)
+    assert np.array_equal(multi, single)
     # a smaller nrhs after a larger one reuses the buffers
     again = f.evaluate_multi(den[:2])
-    assert np.array_equal(again, multi[:2]) if precision == "fp64" else \
-        np.linalg.norm(again - single[:2]) / np.linalg.norm(single[:2]) < 1e-9
+    assert np.array_equal(again, multi[:2])
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
