@@ -221,6 +221,25 @@ int kifmm_field(kifmm_handle h, const double* src_den, double* trg_grad) {
   return KIFMM_OK;
 }
 
+int kifmm_checkpoint_size(kifmm_handle h, int64_t* ndoubles) {
+  if (!h || !ndoubles) return set_err(h, KIFMM_ERR_ARG, "null argument");
+  if (!h->fmm.isHostReady()) return set_err(h, KIFMM_ERR_STATE, "checkpoint size requested before setup");
+  *ndoubles = h->fmm.checkpointSize();
+  return KIFMM_OK;
+}
+
+int kifmm_evaluate_checkpoint_device(kifmm_handle h, const double* d_src_den, double* d_trg_val, double* d_ue_out,
+                                     void* stream) {
+  if (!h) return set_err(nullptr, KIFMM_ERR_ARG, "null handle");
+  return h->fmm.evaluateCheckpoint(d_src_den, d_trg_val, d_ue_out, static_cast<hipStream_t>(stream));
+}
+
+int kifmm_field_checkpoint_device(kifmm_handle h, const double* d_src_den, const double* d_ue_in, double* d_trg_grad,
+                                  void* stream) {
+  if (!h) return set_err(nullptr, KIFMM_ERR_ARG, "null handle");
+  return h->fmm.fieldFromCheckpoint(d_src_den, d_ue_in, d_trg_grad, static_cast<hipStream_t>(stream));
+}
+
 int kifmm_direct_device(kifmm_handle h, const double* d_src_den, int nchk, const double* d_chk_pos, double* d_chk_val,
                         void* stream) {
   if (!h) return set_err(nullptr, KIFMM_ERR_ARG, "null handle");

@@ -94,7 +94,11 @@ struct VlistArgs {
   int np, n, H, sd, td;
   int nsrc, ntrg, ngrp;
   const int* src_node; const int* trg_node; const int* depth; int root_level;
-  const double* deg;           // per source DOF
+  const double* deg;           // per source DOF, scale 2^(l deg) applied to the input samples
+  // per target DOF, scale 2^(l deg_out) applied to the output samples (null:
+  // none); the transposed pass of a kernel whose DOFs have different degrees
+  // scales its output, as the forward pass scales its input
+  const double* deg_out = nullptr;
   const int* shell;            // surface sample -> block-relative (k*np+j)*np+i
   const double2* twiddle;      // n entries exp(-2 pi i m/n)
   const double* UE; int sz_ue;

@@ -644,7 +644,7 @@ void FMM3d::vlist_args(VlistArgs& v, const VGroups& vg) const {
 }
 
 // timed: record the Hadamard's start and end events (forward pass with timing on).
-int FMM3d::run_vlist(VlistArgs& v, hipStream_t s, bool timed) {
+int FMM3d::run_vlist(VlistArgs& v, hipStream_t s, bool timed, const float2* m2l32) {
   int rc;
   if (v.ngrp <= 0) return 0;
   hipEvent_t e0 = timed && timing_ ? events_[PH_COUNT + 1] : nullptr;
@@ -652,7 +652,8 @@ int FMM3d::run_vlist(VlistArgs& v, hipStream_t s, bool timed) {
   if (precision_ != PREC_MIXED) return check(launch_vlist(v, s, e0, e1), "V list");
   if ((rc = check(launch_fft_fwd32(v, dspec32_d_.as<float2>(), s), "V list forward"))) return rc;
   if (e0) (void)hipEventRecord(e0, s);
-  if ((rc = check(launch_hadamard_mixed(v, dspec32_d_.as<float2>(), m2l32_d_.as<float2>(), espec32_d_.as<float2>(), s),
+  if ((rc = check(launch_hadamard_mixed(v, dspec32_d_.as<float2>(), m2l32 ? m2l32 : m2l32_d_.as<float2>(),
+                                        espec32_d_.as<float2>(), s),
                   "V list Hadamard")))
     return rc;
   if (e1) (void)hipEventRecord(e1, s);
@@ -861,7 +862,27 @@ int FMM3d::field(const double* d_src_den, double* d_trg_grad, hipStream_t s) {
 
 // grad: the same far-field pass, then the gradient of the near-field and local
 // expansion kernels at the targets (3 values per target) instead of the values.
-int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t s, bool grad, int nrhs) {
+int64_t FMM3d::checkpointSize() const { return ops_ ? int64_t(tree_.n_nodes) * ops_->dat(UE) : 0; }
+
+int FMM3d::evaluateCheckpoint(const double* d_src_den, double* d_trg_val, double* d_ue_out, hipStream_t s) {
+  if (!setup_done_) return fail(-3, "evaluate called before setup");
+  if (!d_ue_out) return fail(-2, "null checkpoint pointer");
+  return evaluate_impl(d_src_den, d_trg_val, s, false, 1, nullptr, d_ue_out);
+}
+
+int FMM3d::fieldFromCheckpoint(const double* d_src_den, const double* d_ue_in, double* d_trg_grad, hipStream_t s) {
+  if (!setup_done_) return fail(-3, "field called before setup");
+  if (kt_ != 111) return fail(-2, "the field pass supports the Laplace single layer (kernel 111) only");
+  if (!d_ue_in) return fail(-2, "null checkpoint pointer");
+  if (tgrad_d_.bytes() < sizeof(double) * 3 * size_t(std::max<int64_t>(1, nt_))) {
+    int rc = check(tgrad_d_.alloc(sizeof(double) * 3 * size_t(std::max<int64_t>(1, nt_))), "alloc field");
+    if (rc) return rc;
+  }
+  return evaluate_impl(d_src_den, d_trg_grad, s, true, 1, d_ue_in, nullptr);
+}
+
+int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t s, bool grad, int nrhs,
+                         const double* ue_in, double* ue_out) {
   if ((ns_ > 0 && !d_src_den) || (nt_ > 0 && !d_trg_val)) return fail(-2, "null density or value pointer");
   const int R = nrhs;
   if (R < 1 || R > rhs_cap_ || (R > 1 && (grad || rhs_steps_n_ != R))) return fail(-6, "right-hand sides not prepared");
@@ -878,7 +899,7 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
   mark(PH_INI);
   // zero the work vectors (:425-438)
   const int64_t NR = int64_t(N) * R;  // node rows over the right-hand sides
-  if ((rc = check(launch_fill(ue_d_.as<double>(), NR * op.dat(UE), 0.0, s), "zero UE"))) return rc;
+  if (!ue_in && (rc = check(launch_fill(ue_d_.as<double>(), NR * op.dat(UE), 0.0, s), "zero UE"))) return rc;
   // mixed path: UC is read only for the contributor leaves, whose S2M task
   // stores it whole (the fused up pass forms no UC for parents)
   if (!mixed && (rc = check(launch_fill(uc_d_.as<double>(), NR * op.dat(UC), 0.0, s), "zero UC")))
@@ -951,14 +972,22 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
   sa.leaf = mixed ? s2m_leaf_d_.as<int4>() : nullptr;
   sa.sam = sam_d_[UC].as<double>(); sa.n_sam = op.n_sam[UC];
   sa.out = uc_d_.as<double>(); sa.sz_out = op.dat(UC);
-  if ((rc = check(mixed ? launch_surface_mixed(kt_, sa, pu, s) : launch_surface(kt_, sa, pu, s), "S2M"))) return rc;
-  if ((rc = run_steps(R > 1 ? up_steps_r_ : up_steps_))) return rc;
+  if (!ue_in) {
+    if ((rc = check(mixed ? launch_surface_mixed(kt_, sa, pu, s) : launch_surface(kt_, sa, pu, s), "S2M")))
+      return rc;
+    if ((rc = run_steps(R > 1 ? up_steps_r_ : up_steps_))) return rc;
+  }
+  if (ue_out &&
+      (rc = check(hipMemcpyAsync(ue_out, ue_d_.as<double>(), sizeof(double) * size_t(N) * op.dat(UE),
+                                 hipMemcpyDeviceToDevice, s), "checkpoint UE")))
+    return rc;
+  const double* ue_src = ue_in ? ue_in : ue_d_.as<double>();
 
   // V list (:1007-1059)
   mark(PH_VLIST);
   VlistArgs v{};
   vlist_args(v, vfwd_);
-  v.UE = ue_d_.as<double>(); v.sz_ue = op.dat(UE);
+  v.UE = ue_src; v.sz_ue = op.dat(UE);
   v.DC = dc_d_.as<double>(); v.sz_dc = op.dat(DC);
   v.nrhs = R;
   v.nstride = N;
@@ -987,7 +1016,7 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
   na.geom = geom_d_.as<double4>();
   na.sam_ue = sam_d_[UE].as<double>(); na.n_ue = op.n_sam[UE];
   na.sam_de = sam_d_[DE].as<double>(); na.n_de = op.n_sam[DE];
-  na.UE = ue_d_.as<double>(); na.sz_ue = op.dat(UE);
+  na.UE = ue_src; na.sz_ue = op.dat(UE);
   na.DE = de_d_.as<double>(); na.sz_de = op.dat(DE);
   na.tpos = tpos_d_.as<double>(); na.tval = tval_d_.as<double>();
   na.nrhs = R;

@@ -108,6 +108,13 @@ class FMM3d {
   // Gradient of the evaluated potential at the targets (3 values per target,
   // caller's order); Laplace single layer only.
   int field(const double* d_src_den, double* d_trg_grad, hipStream_t stream);
+  // Upward equivalent densities as a checkpoint: evaluate() that also copies UE
+  // (checkpointSize() doubles, node-major) to d_ue_out once the up pass is
+  // done, and field() that takes UE from such a checkpoint instead of running
+  // the up pass again (the densities must be those of the checkpoint).
+  int64_t checkpointSize() const;
+  int evaluateCheckpoint(const double* d_src_den, double* d_trg_val, double* d_ue_out, hipStream_t stream);
+  int fieldFromCheckpoint(const double* d_src_den, const double* d_ue_in, double* d_trg_grad, hipStream_t stream);
   // FP64 direct sum at the given targets (device pointers, caller's order);
   // the partial-sum workspace is the handle's, grown on demand.
   int direct(const double* d_src_den, int nchk, const double* d_chk_pos, double* d_chk_val,
@@ -142,7 +149,10 @@ class FMM3d {
  private:
   int fail(int code, const std::string& msg) { err_ = msg; return code; }
   int check(hipError_t e, const char* what);
-  int evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t s, bool grad, int nrhs = 1);
+  // ue_in: UE from a checkpoint (the up pass is skipped); ue_out: UE copied
+  // there after the up pass
+  int evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t s, bool grad, int nrhs = 1,
+                    const double* ue_in = nullptr, double* ue_out = nullptr);
   int run_graph(const double* d_src_den, double* d_trg_val, hipStream_t s, int nrhs);
   int reserve_rhs(int nrhs);
   int prepare_rhs_steps(int nrhs);
@@ -153,7 +163,8 @@ class FMM3d {
                     bool need_evtr, VGroups& vg);
   int upload_vgroups(VGroups& vg);
   void vlist_args(VlistArgs& v, const VGroups& vg) const;
-  int run_vlist(VlistArgs& v, hipStream_t s, bool timed);
+  // m2l32: FP32 translation table of the mixed path (null: the forward one)
+  int run_vlist(VlistArgs& v, hipStream_t s, bool timed, const float2* m2l32 = nullptr);
   int build_adjoint();
 
   int kt_ = 111, np_ = 6, pts_max_ = 64, max_level_ = 30, root_level_ = 1, precision_ = PREC_FP64;
@@ -223,6 +234,7 @@ class FMM3d {
   DeviceBuffer dspec_d_, espec_d_, tgrad_d_;
   DeviceBuffer nt_chunks_d_, nt_items_d_, ue_t_tasks_d_, de_t_tasks_d_, surf_t_items_d_, tbeg_d_, tnum_d_;
   DeviceBuffer m2mT_d_, pupT_d_, pdnT_d_, l2lT_d_;
+  DeviceBuffer m2lT_d_, m2lT32_d_, deg0_d_;  // adjoint V list of the Stokes equivalent kernel
   DeviceBuffer wup_d_;  // mixed path: P_up [M_0 ... M_7]
   DeviceBuffer direct_work_d_;  // direct sum: partial sums of the source chunks
   DeviceBuffer m2l32_d_, dspec32_d_, espec32_d_;  // mixed path: FP32 spectra

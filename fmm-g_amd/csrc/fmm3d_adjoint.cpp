@@ -13,10 +13,15 @@
 //   near     phi += K sigma + K_mm UE + K_mm DE   sigma_bar, UEbar, DEbar += transposes
 //
 // The V-list step uses the forward machinery with the roles of the two boxes
-// swapped: for a symmetric equivalent kernel (K_mm(x, y) = K_mm(y, x)^T and even,
-// mm kernels 111 and 511) the transpose of the translation B <- A is the
-// translation A <- B on the same sample grid, and the homogeneous level scale
-// is a scalar shared by A and B (same level).
+// swapped: the transpose of the translation B <- A is a translation A <- B on
+// the same sample grid with the kernel K'(r) = K_mm(-r)^T.  For the even,
+// symmetric equivalent kernels (111, 511) that is the forward table at the
+// negated offset, and the homogeneous level scale is a scalar shared by A and
+// B (same level).  For the Stokes equivalent kernel 301 (sd 4, td 3; the
+// velocity part even, the pressure-doublet column odd) it is the forward table
+// with the DOFs transposed and the odd column negated (3 -> 4 DOFs), and the
+// level scale 2^(l deg), whose degrees differ per DOF (1, 1, 1, 2), moves from
+// the input to the output of the pass.
 #include <algorithm>
 #include <cmath>
 
@@ -32,9 +37,9 @@ int FMM3d::build_adjoint() {
   const Tree& t = tree_;
   const Operators& op = *ops_;
   const int N = t.n_nodes, sd = op.sd;
-  if (op.kt_mm != 111 && op.kt_mm != 511)
-    return fail(-2, "the adjoint pass needs a symmetric equivalent kernel (111 or 511); kernel " +
-                        std::to_string(kt_) + " uses " + std::to_string(op.kt_mm));
+  if (op.kt_mm != 111 && op.kt_mm != 511 && op.kt_mm != 301)
+    return fail(-2, "no adjoint pass for kernel " + std::to_string(kt_) + " (equivalent kernel " +
+                        std::to_string(op.kt_mm) + ")");
 
   // --- source-centric near field: per source leaf, the target leaves whose
   // near lists hold it as exact sources, the S2L boxes that read it, and its
@@ -174,14 +179,36 @@ int FMM3d::build_adjoint() {
           l[(size_t(c) * dc + i) * de + j] = op.l2l[size_t(c) * dc * de + size_t(j) * dc + i];
     KIFMM_UP(l2lT_d_, l, "upload L2L^T");
   }
-#undef KIFMM_UP
   if ((rc = upload_vgroups(vadj_))) return rc;
+  if (op.kt_mm == 301) {
+    // transposed translation spectra: [slot][i * sd + j] = s_j T[slot][j * td + i]
+    // (s = +1, +1, +1, -1: the odd pressure-doublet column), FP64 and FP32
+    const int sd4 = op.sd, td3 = op.td, H = op.H;
+    std::vector<double> t(op.m2l.size());
+    for (int slot = 0; slot < 343; slot++)
+      for (int j = 0; j < sd4; j++)
+        for (int i = 0; i < td3; i++) {
+          const double sg = j == 3 ? -1.0 : 1.0;
+          const double* src = op.m2l.data() + ((size_t(slot) * sd4 * td3 + size_t(j) * td3 + i) * H) * 2;
+          double* dst = t.data() + ((size_t(slot) * sd4 * td3 + size_t(i) * sd4 + j) * H) * 2;
+          for (int f = 0; f < 2 * H; f++) dst[f] = sg * src[f];
+        }
+    KIFMM_UP(m2lT_d_, t, "upload transposed M2L");
+    if (precision_ == PREC_MIXED) {
+      std::vector<float> t32(t.begin(), t.end());
+      KIFMM_UP(m2lT32_d_, t32, "upload transposed M2L (FP32)");
+    }
+    std::vector<double> z(size_t(op.td), 0.0);
+    KIFMM_UP(deg0_d_, z, "upload degrees");
+  }
+#undef KIFMM_UP
   // spectra buffers hold the larger of the two passes (a captured forward pass
   // holds the old addresses)
   drop_graph();
   {
-    const size_t nsrc = std::max(vfwd_.src_node.size(), vadj_.src_node.size()) * op.sd;
-    const size_t ntrg = std::max(vfwd_.trg_node.size(), vadj_.trg_node.size()) * op.td;
+    const size_t dmax = size_t(std::max(op.sd, op.td));
+    const size_t nsrc = std::max(vfwd_.src_node.size(), vadj_.src_node.size()) * dmax;
+    const size_t ntrg = std::max(vfwd_.trg_node.size(), vadj_.trg_node.size()) * dmax;
     if (precision_ == PREC_FP64) {
       if (dspec_d_.bytes() < sizeof(double2) * nsrc * op.H &&
           (rc = check(dspec_d_.alloc(sizeof(double2) * nsrc * op.H), "alloc V spectra")))
@@ -296,7 +323,16 @@ int FMM3d::evaluateAdjoint(const double* d_trg_in, double* d_src_out, hipStream_
   vlist_args(v, vadj_);
   v.UE = dc_d_.as<double>(); v.sz_ue = op.dat(DC);
   v.DC = ue_d_.as<double>(); v.sz_dc = op.dat(UE);
-  if ((rc = run_vlist(v, s, false))) return rc;
+  const float2* m2l32 = nullptr;
+  if (op.kt_mm == 301) {  // 3 DOFs in, 4 out; the level scale on the output
+    v.sd = op.td;
+    v.td = op.sd;
+    v.m2l = m2lT_d_.as<double2>();
+    m2l32 = m2lT32_d_.as<float2>();
+    v.deg = deg0_d_.as<double>();
+    v.deg_out = deg_d_.as<double>();
+  }
+  if ((rc = run_vlist(v, s, false, m2l32))) return rc;
 
   if ((rc = run_steps(up_t_steps_))) return rc;
 

@@ -576,7 +576,9 @@ __global__ __launch_bounds__(64) void k_fft_inv(VlistArgs v, const C2* espec) {
   __syncthreads();
   for (int q = lane; q < NSAM; q += 64) qof[v.shell[q]] = q;
   double* dc = v.DC + drow * v.sz_dc;
+  const int lvl = v.depth[node] + v.root_level;
   for (int i = 0; i < v.td; i++) {
+    const double oscale = v.deg_out ? ldexp(1.0, int(lvl * v.deg_out[i])) : 1.0;
     const C2* in = espec + (int64_t(slot) * v.td + i) * (N * N * HH);
     __syncthreads();
     // along z (a), only block rows a' in [0,np); line (kb, kc).  When the n*h
@@ -672,7 +674,7 @@ __global__ __launch_bounds__(64) void k_fft_inv(VlistArgs v, const C2* espec) {
           const double2 tw = w[((c + O) * kc) % N];
           val += 2.0 * (yy[kc].x * tw.x - yy[kc].y * tw.y);
         }
-        if (qc[c] >= 0) dc[qc[c] * v.td + i] = old[c] + val;
+        if (qc[c] >= 0) dc[qc[c] * v.td + i] = old[c] + (v.deg_out ? oscale * val : val);
       }
     }
   }
@@ -719,6 +721,7 @@ hipError_t launch_vlist(const VlistArgs& v, hipStream_t s, hipEvent_t e0, hipEve
   if (v.sd == 1 && v.td == 1) k_hadamard<1, 1><<<grid, dim3(64), 0, s>>>(v, nrm);
   else if (v.sd == 4 && v.td == 3) k_hadamard<4, 3><<<grid, dim3(64), 0, s>>>(v, nrm);
   else if (v.sd == 3 && v.td == 3) k_hadamard<3, 3><<<grid, dim3(64), 0, s>>>(v, nrm);
+  else if (v.sd == 3 && v.td == 4) k_hadamard<3, 4><<<grid, dim3(64), 0, s>>>(v, nrm);
   else return hipErrorInvalidValue;
   e = hipGetLastError();
   if (e != hipSuccess) return e;

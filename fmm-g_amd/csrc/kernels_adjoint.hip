@@ -255,7 +255,7 @@ hipError_t launch_near_grad(int kt, const NearArgs& a, const KParams<double>& pu
 }
 
 // ---------------------------------------------------------------------------
-#define KIFMM_ADJ_KERNELS(X) X(111) X(121) X(312) X(322) X(511) X(521)
+#define KIFMM_ADJ_KERNELS(X) X(111) X(121) X(311) X(312) X(321) X(322) X(511) X(521)
 
 hipError_t launch_near_adjoint(int kt, const NearTArgs& a, const KParams<double>& pu, hipStream_t s) {
   if (a.nchunks <= 0) return hipSuccess;
@@ -273,6 +273,7 @@ hipError_t launch_surface_adjoint(int km, const SurfTArgs& a, const KParams<doub
   if (a.n_sam > 2 * kSurfT) return hipErrorInvalidValue;
   switch (km) {
     case 111: k_surf_t<111><<<dim3(unsigned(a.ntasks)), dim3(kSurfT), 0, s>>>(a, pm); break;
+    case 301: k_surf_t<301><<<dim3(unsigned(a.ntasks)), dim3(kSurfT), 0, s>>>(a, pm); break;
     case 511: k_surf_t<511><<<dim3(unsigned(a.ntasks)), dim3(kSurfT), 0, s>>>(a, pm); break;
     default: return hipErrorInvalidValue;
   }

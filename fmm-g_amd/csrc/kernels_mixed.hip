@@ -818,6 +818,7 @@ hipError_t launch_hadamard_mixed(const VlistArgs& v, const float2* dspec, const 
   }
   else if (v.sd == 4 && v.td == 3) k_hadamard_mx<4, 3><<<grid, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm);
   else if (v.sd == 3 && v.td == 3) k_hadamard_mx<3, 3><<<grid, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm);
+  else if (v.sd == 3 && v.td == 4) k_hadamard_mx<3, 4><<<grid, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }

@@ -48,6 +48,9 @@ SIGNATURES = {
     "kifmm_field": (C.c_int, [H, P_D, P_D]),
     "kifmm_field_device": (C.c_int, [H, C.c_void_p, C.c_void_p, C.c_void_p]),
     "kifmm_evaluate_adjoint_device": (C.c_int, [H, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kifmm_checkpoint_size": (C.c_int, [H, P_L]),
+    "kifmm_evaluate_checkpoint_device": (C.c_int, [H, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kifmm_field_checkpoint_device": (C.c_int, [H, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "kifmm_direct_device": (C.c_int, [H, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "kifmm_check": (C.c_int, [H, P_D, P_D, C.c_int, C.c_uint64, P_D]),
     "kifmm_tree_info": (C.c_int, [H, P_L]),
@@ -270,6 +273,28 @@ class FMM3d:
         self._chk(lib().kifmm_field_device(self._h, _dev(d_src_den, self.n_src * self.src_dof, "densities"),
                                            _dev(d_trg_grad, self.n_trg * 3, "gradients"),
                                            _stream(stream)))
+
+    def checkpoint_size(self):
+        """Doubles of an upward-equivalent-density checkpoint."""
+        n = C.c_int64()
+        self._chk(lib().kifmm_checkpoint_size(self._h, C.byref(n)))
+        return n.value
+
+    def evaluate_checkpoint_device(self, d_src_den, d_trg_val, d_ue, stream=None):
+        """evaluate_device that also writes the upward equivalent densities to
+        d_ue (checkpoint_size() float64 values)."""
+        self._chk(lib().kifmm_evaluate_checkpoint_device(
+            self._h, _dev(d_src_den, self.n_src * self.src_dof, "densities"),
+            _dev(d_trg_val, self.n_trg * self.trg_dof, "values"), _dev(d_ue, self.checkpoint_size(), "checkpoint"),
+            _stream(stream)))
+
+    def field_checkpoint_device(self, d_src_den, d_ue, d_trg_grad, stream=None):
+        """field_device with the up pass taken from a checkpoint of the same
+        densities."""
+        self._chk(lib().kifmm_field_checkpoint_device(
+            self._h, _dev(d_src_den, self.n_src * self.src_dof, "densities"),
+            _dev(d_ue, self.checkpoint_size(), "checkpoint"), _dev(d_trg_grad, self.n_trg * 3, "gradients"),
+            _stream(stream)))
 
     def direct_device(self, d_src_den, d_chk_pos, d_chk_val, stream=None):
         """FP64 direct sum at the check positions (device tensors)."""

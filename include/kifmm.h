@@ -108,9 +108,8 @@ int kifmm_evaluate_multi_device(kifmm_handle h, const double *d_src_den, double 
  * kifmm_evaluate applies (the transposed FMM the reference lacks: its
  * evaluate, fmm3d_eval_mpi.cpp:396-1515, has no adjoint; every forward step is
  * transposed in reverse order).  trg_in: trgDOF per target, src_out: srcDOF
- * per source, caller's order, src_out overwritten.  Needs a symmetric
- * equivalent kernel (kernels 111, 121, 312, 322, 511, 521); KIFMM_ERR_ARG
- * otherwise.  Host pointers, synchronous. */
+ * per source, caller's order, src_out overwritten.  All eight FMM kernels
+ * (111, 121, 311, 312, 321, 322, 511, 521).  Host pointers, synchronous. */
 int kifmm_evaluate_adjoint(kifmm_handle h, const double *trg_in, double *src_out);
 
 /* Same with device pointers, enqueued on `stream` (the first call on a handle
@@ -126,6 +125,20 @@ int kifmm_field(kifmm_handle h, const double *src_den, double *trg_grad);
 
 /* Same with device pointers, enqueued on `stream`. */
 int kifmm_field_device(kifmm_handle h, const double *d_src_den, double *d_trg_grad, void *stream);
+
+/* Checkpoint of the upward equivalent densities (the UE vectors of
+ * fmm3d_eval_mpi.cpp:480-619, kept for reuse by a gradient pass; no
+ * reference counterpart).  kifmm_checkpoint_size gives the doubles a
+ * checkpoint holds.  kifmm_evaluate_checkpoint_device is
+ * kifmm_evaluate_device that also copies UE to d_ue_out once the up pass is
+ * done; kifmm_field_checkpoint_device is kifmm_field_device that reads UE from
+ * such a checkpoint instead of running the up pass again (d_src_den must be
+ * the checkpointed densities).  Device pointers, enqueued on `stream`. */
+int kifmm_checkpoint_size(kifmm_handle h, int64_t *ndoubles);
+int kifmm_evaluate_checkpoint_device(kifmm_handle h, const double *d_src_den, double *d_trg_val, double *d_ue_out,
+                                     void *stream);
+int kifmm_field_checkpoint_device(kifmm_handle h, const double *d_src_den, const double *d_ue_in, double *d_trg_grad,
+                                  void *stream);
 
 /* FP64 direct sum at nchk device-resident targets with the user kernel
  * (Dense3d_MPI::evaluate, dense3d_mpi.cpp:75-141). */
