@@ -361,6 +361,19 @@ const char* kifmm_phase_name(int phase) {
   return (phase >= 0 && phase < kifmm::PH_TIMES) ? kifmm::kPhaseNames[phase] : "";
 }
 
+int kifmm_set_tree_builder(kifmm_handle h, int on_device) {
+  if (!h) return set_err(nullptr, KIFMM_ERR_ARG, "null handle");
+  h->fmm.setTreeOnDevice(on_device != 0);
+  return KIFMM_OK;
+}
+
+int kifmm_setup_times(kifmm_handle h, double* ms, int n) {
+  if (!h || !ms) return set_err(h, KIFMM_ERR_ARG, "null argument");
+  const auto& s = h->fmm.setupMs();
+  for (int i = 0; i < n && i < KIFMM_NSETUP; i++) ms[i] = s[size_t(i)];
+  return KIFMM_OK;
+}
+
 int kifmm_work_counts(kifmm_handle h, double* out, int n) {
   if (!h || !out) return set_err(h, KIFMM_ERR_ARG, "null argument");
   if (!h->fmm.isHostReady()) return set_err(h, KIFMM_ERR_STATE, "work counts requested before setup");

@@ -3,11 +3,13 @@
 #include "fmm3d.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <map>
 
 #include "kernel3d.hpp"
+#include "tree_device.hpp"
 
 namespace kifmm {
 
@@ -101,15 +103,22 @@ int FMM3d::setPointsBatch(int nprob, const int64_t* ns, const double* spos, cons
 // ---------------------------------------------------------------------------
 int FMM3d::setup() {
   drop_graph();
-  int rc = setupHost();
+  int rc = setup_prepare(tree_on_device_);
   if (rc) return rc;
+  const auto t0 = std::chrono::steady_clock::now();
   rc = upload_state();
   if (rc) return rc;
+  setup_ms_[SU_UPLOAD] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   setup_done_ = true;
   return 0;
 }
 
-int FMM3d::setupHost() {
+int FMM3d::setupHost() { return setup_prepare(false); }
+
+int FMM3d::setup_prepare(bool device_tree) {
+  using clk = std::chrono::steady_clock;
+  auto ms_since = [](clk::time_point a) { return std::chrono::duration<double, std::milli>(clk::now() - a).count(); };
+  setup_ms_.fill(0.0);
   setup_done_ = false;
   host_ready_ = false;
   if (!kernel_known(kt_)) return fail(-2, "unknown kernel type " + std::to_string(kt_));
@@ -120,7 +129,20 @@ int FMM3d::setupHost() {
   if (np_ < 3 || np_ > 8) return fail(-2, "np must lie in [3,8]");
   if (precision_ != PREC_FP64 && precision_ != PREC_MIXED) return fail(-2, "unknown precision mode");
   int rc = 0;
-  if (prob_ns_.empty()) {
+  auto t0 = clk::now();
+  if (device_tree) {
+    // the same tree, built on the GPU (one launch sequence for all problems)
+    const int P = numProblems();
+    const int64_t one_s = ns_, one_t = nt_;
+    const int64_t* pns = prob_ns_.empty() ? &one_s : prob_ns_.data();
+    const int64_t* pnt = prob_ns_.empty() ? &one_t : prob_nt_.data();
+    double tms[2] = {0, 0};
+    rc = build_tree_device(P, pns, spos_.data(), pnt, tpos_.data(), ctr_, root_level_, pts_max_, max_level_, tree_,
+                           &err_, nullptr, tms);
+    if (rc) return rc;
+    setup_ms_[SU_TREE_DEVICE] = tms[0];
+    setup_ms_[SU_TREE_COPY] = tms[1];
+  } else if (prob_ns_.empty()) {
     rc = build_tree(spos_.data(), ns_, tpos_.data(), nt_, ctr_, root_level_, pts_max_, max_level_, tree_, &err_);
     if (rc) return rc;
   } else {
@@ -140,14 +162,19 @@ int FMM3d::setupHost() {
       if (prc[p]) return fail(prc[p], "problem " + std::to_string(p) + ": " + perr[p]);
     merge_trees(parts, so, to, tree_);
   }
+  setup_ms_[SU_TREE] = ms_since(t0);
+  t0 = clk::now();
   std::string e;
   std::vector<double> mmc = coefs_;
   if (kt_ == 312 || kt_ == 322) mmc.clear();  // fmm3d_setup_mpi.cpp:119,121
   if (mmc.empty()) mmc = {1.0, 0.25};
   ops_ = get_operators(km, mmc, np_, &e);
   if (!ops_) return fail(-2, e);
+  setup_ms_[SU_OPS] = ms_since(t0);
+  t0 = clk::now();
   rc = build_tasks();
   if (rc) return rc;
+  setup_ms_[SU_TASKS] = ms_since(t0);
   rhs_steps_n_ = 1;
   up_steps_r_.clear(); dn_steps_r_.clear(); lvl_idx_r_h_.clear();
   host_ready_ = true;

@@ -91,7 +91,14 @@ class FMM3d {
   // Host half of setup(): octree, lists, operator tables and task tables, no
   // device calls (usable on a machine without a GPU).
   int setupHost();
+  // setup() builds the octree and lists on the GPU (default) or on the host;
+  // both give the same tree.
   int setup();
+  void setTreeOnDevice(bool on) { tree_on_device_ = on; }
+  // Wall times of the last setup, milliseconds: tree (total), of which device
+  // build and copy-back (device builder only), operators, task tables, upload.
+  enum SetupPart { SU_TREE = 0, SU_TREE_DEVICE, SU_TREE_COPY, SU_OPS, SU_TASKS, SU_UPLOAD, SU_COUNT };
+  const std::array<double, SU_COUNT>& setupMs() const { return setup_ms_; }
   // Device pointers, caller's point order (srcDOF / trgDOF interleaved).
   int evaluate(const double* d_src_den, double* d_trg_val, hipStream_t stream);
   // nrhs right-hand sides in one pass: d_src_den holds nrhs density vectors and
@@ -148,6 +155,7 @@ class FMM3d {
 
  private:
   int fail(int code, const std::string& msg) { err_ = msg; return code; }
+  int setup_prepare(bool device_tree);
   int check(hipError_t e, const char* what);
   // ue_in: UE from a checkpoint (the up pass is skipped); ue_out: UE copied
   // there after the up pass
@@ -173,7 +181,8 @@ class FMM3d {
   int64_t ns_ = 0, nt_ = 0;
   std::vector<double> spos_, snor_, tpos_;
   std::vector<int64_t> prob_ns_, prob_nt_;  // forest: points per problem (empty: one problem)
-  bool setup_done_ = false, host_ready_ = false, timing_ = false;
+  bool setup_done_ = false, host_ready_ = false, timing_ = false, tree_on_device_ = true;
+  std::array<double, SU_COUNT> setup_ms_{};
   std::string err_;
   Tree tree_;
   std::shared_ptr<const Operators> ops_;

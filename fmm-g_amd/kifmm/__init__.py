@@ -24,6 +24,7 @@ H = C.c_void_p
 
 PREC = {"fp64": 0, "mixed": 1}
 PHASES = ["EvlIni", "EvlUpwComp", "EvlVList", "EvlXList", "EvlCombine", "EvlUList", "EvlFinalize", "VListHadamard"]
+SETUP_PARTS = ["tree", "tree_device", "tree_copy", "operators", "tasks", "upload"]
 WORK = ["near_pairs", "near_exact_pairs", "s2m_pairs", "s2l_pairs", "vlist_pairs", "n_vsrc",
         "n_vtrg", "n_vgrp", "up_cols", "m2m_cols", "dn_cols", "l2l_cols"]
 
@@ -61,6 +62,8 @@ SIGNATURES = {
     "kifmm_phase_times": (C.c_int, [H, P_F, C.c_int]),
     "kifmm_phase_name": (C.c_char_p, [C.c_int]),
     "kifmm_work_counts": (C.c_int, [H, P_D, C.c_int]),
+    "kifmm_set_tree_builder": (C.c_int, [H, C.c_int]),
+    "kifmm_setup_times": (C.c_int, [H, P_D, C.c_int]),
     "kifmm_options_lookup": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int]),
     "kifmm_last_error": (C.c_char_p, [H]),
 }
@@ -358,3 +361,13 @@ class FMM3d:
         w = np.zeros(len(WORK))
         self._chk(lib().kifmm_work_counts(self._h, _pd(w), len(WORK)))
         return dict(zip(WORK, w.tolist()))
+
+    def set_tree_builder(self, on_device=True):
+        """setup() builds the octree and lists on the GPU (default) or on the host."""
+        self._chk(lib().kifmm_set_tree_builder(self._h, 1 if on_device else 0))
+
+    def setup_times(self):
+        """Wall times of the last setup, milliseconds, by part."""
+        ms = np.zeros(len(SETUP_PARTS))
+        self._chk(lib().kifmm_setup_times(self._h, _pd(ms), len(SETUP_PARTS)))
+        return dict(zip(SETUP_PARTS, ms.tolist()))

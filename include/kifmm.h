@@ -160,6 +160,18 @@ int kifmm_get_tree(kifmm_handle h, int *parent, int *child, int *depth, int *pat
 /* kind: 0 U, 1 V, 2 W, 3 X (Node::Unodes()..Xnodes()); off has n_nodes+1 entries. */
 int kifmm_get_list(kifmm_handle h, int kind, int64_t *off, int *lst);
 
+/* Where kifmm_setup builds the octree and lists (the tree part of
+ * Let3d_MPI::setup, let3d_mpi.cpp:611-726, 1209-1876): on_device != 0 (the
+ * default) on the GPU, 0 on the host.  Both give the same tree, bit for bit;
+ * kifmm_setup_host always uses the host. */
+int kifmm_set_tree_builder(kifmm_handle h, int on_device);
+/* Wall times of the last setup in milliseconds (FMM3d_MPI::setup,
+ * fmm3d_setup_mpi.cpp:41-565, has PETSc events instead): tree and lists
+ * (total), of which the device build and its copy back to the host (device
+ * builder), operator tables, task tables, upload.  ms has KIFMM_NSETUP slots. */
+#define KIFMM_NSETUP 6
+int kifmm_setup_times(kifmm_handle h, double *ms, int n);
+
 /* Operator tables of MatMgnt3d_MPI (matmgnt3d_mpi.cpp:107-431), column-major
  * FP64: which 0 = UC->UE pinv, 1 = DC->DE pinv, 2 = 8 M2M, 3 = 8 L2L,
  * 4 = M2L spectra [343][sd*td][H] complex.  With out == NULL only *count
