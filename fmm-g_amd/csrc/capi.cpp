@@ -246,6 +246,12 @@ int kifmm_direct_device(kifmm_handle h, const double* d_src_den, int nchk, const
   return h->fmm.direct(d_src_den, nchk, d_chk_pos, d_chk_val, static_cast<hipStream_t>(stream));
 }
 
+int kifmm_direct_device_noalloc(kifmm_handle h, const double* d_src_den, int nchk, const double* d_chk_pos,
+                                double* d_chk_val, void* stream) {
+  if (!h) return set_err(nullptr, KIFMM_ERR_ARG, "null handle");
+  return h->fmm.directNoAlloc(d_src_den, nchk, d_chk_pos, d_chk_val, static_cast<hipStream_t>(stream));
+}
+
 int kifmm_check(kifmm_handle h, const double* src_den, const double* trg_val, int numchk, uint64_t seed,
                 double* rerr) {
   if (!h) return set_err(nullptr, KIFMM_ERR_ARG, "null handle");

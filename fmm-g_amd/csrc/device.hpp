@@ -190,4 +190,10 @@ hipError_t launch_direct(int kt, const KParams<double>& p, int64_t ns, const dou
                          const int* sperm, const double* sden, int nt, const double* tpos, const DirectPlan& d,
                          double* work, double* tval, hipStream_t s);
 
+// Library-held scratch of kScratchDoubles doubles per GPU (a static device
+// array, no allocation): its device address on the current GPU, looked up on
+// the first call for that GPU and kept.
+constexpr int64_t kScratchDoubles = int64_t(1) << 21;  // 16 MiB
+hipError_t scratch_address(double** out);
+
 }  // namespace kifmm

@@ -1099,6 +1099,24 @@ int FMM3d::direct(const double* d_src_den, int nchk, const double* d_chk_pos, do
   return direct_with(d_src_den, nchk, d_chk_pos, d_chk_val, d, direct_work_d_.as<double>(), s);
 }
 
+int FMM3d::directNoAlloc(const double* d_src_den, int nchk, const double* d_chk_pos, double* d_chk_val,
+                         hipStream_t s) {
+  if (!setup_done_) return fail(-3, "direct called before setup");
+  if (nchk < 0) return fail(-2, "negative check-target count");
+  if (nchk == 0) return 0;
+  if ((ns_ > 0 && !d_src_den) || !d_chk_pos || !d_chk_val) return fail(-2, "null density, position or value pointer");
+  const int TD = trgDOF();
+  if (int64_t(nchk) * TD > kScratchDoubles)
+    return fail(-2, "nchk * trgDOF = " + std::to_string(int64_t(nchk) * TD) + " exceeds the scratch capacity of " +
+                        std::to_string(kScratchDoubles) + " values");
+  double* work = nullptr;
+  int rc = check(scratch_address(&work), "scratch address");
+  if (rc) return rc;
+  DirectPlan d = plan_direct(ns_, nchk, TD, kScratchDoubles);
+  if (d.nchunk <= 0) return fail(-6, "direct sum plan failed");
+  return direct_with(d_src_den, nchk, d_chk_pos, d_chk_val, d, work, s);
+}
+
 int FMM3d::direct_with(const double* d_src_den, int nchk, const double* d_chk_pos, double* d_chk_val,
                        const DirectPlan& d, double* work, hipStream_t s) {
   const KParams<double> pu = make_params(kt_, coefs_.data());

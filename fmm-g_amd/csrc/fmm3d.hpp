@@ -130,6 +130,11 @@ class FMM3d {
   // d.work_doubles doubles (allocates nothing).
   int direct_with(const double* d_src_den, int nchk, const double* d_chk_pos, double* d_chk_val,
                   const DirectPlan& d, double* work, hipStream_t stream);
+  // direct() without any allocation: the partial sums go to the library's
+  // fixed per-GPU scratch (kScratchDoubles doubles); nchk * trgDOF beyond it is
+  // rejected.  Calls that overlap on one GPU share the scratch and race.
+  int directNoAlloc(const double* d_src_den, int nchk, const double* d_chk_pos, double* d_chk_val,
+                    hipStream_t stream);
 
   const Tree& tree() const { return tree_; }
   const Operators* ops() const { return ops_.get(); }

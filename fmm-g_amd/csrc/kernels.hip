@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "device.hpp"
@@ -892,6 +893,38 @@ hipError_t launch_direct(int kt, const KParams<double>& p, int64_t ns, const dou
   const int64_t n = int64_t(nt) * trg_dof(kt);
   k_direct_sum<<<dim3(unsigned((n + 255) / 256)), dim3(256), 0, s>>>(work, d.nchunk, n, tval);
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Fixed scratch for callers that cannot allocate: the direct sum's partial
+// sums live in this array, one copy per GPU, held until the process ends.
+// Every slot a call reads is written earlier in the same call, so nothing is
+// cleared; calls that overlap on one GPU share it and race.
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t scratch_address(double** out) {
+  *out = nullptr;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    // the array's device address on the current GPU (the host-side name is
+    // only a stand-in and must never reach a kernel)
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
 }
 
 }  // namespace kifmm

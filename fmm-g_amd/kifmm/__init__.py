@@ -53,6 +53,7 @@ SIGNATURES = {
     "kifmm_evaluate_checkpoint_device": (C.c_int, [H, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "kifmm_field_checkpoint_device": (C.c_int, [H, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "kifmm_direct_device": (C.c_int, [H, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "kifmm_direct_device_noalloc": (C.c_int, [H, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
     "kifmm_check": (C.c_int, [H, P_D, P_D, C.c_int, C.c_uint64, P_D]),
     "kifmm_tree_info": (C.c_int, [H, P_L]),
     "kifmm_get_tree": (C.c_int, [H] + [P_I] * 11),
@@ -308,6 +309,17 @@ class FMM3d:
                                             _dev(d_chk_pos, 3 * n, "check positions"),
                                             _dev(d_chk_val, n * self.trg_dof, "check values"),
                                             _stream(stream)))
+
+    def direct_device_noalloc(self, d_src_den, d_chk_pos, d_chk_val, stream=None):
+        """direct_device with the library's fixed per-GPU scratch: allocates
+        nothing (check targets * trg_dof <= 2^21)."""
+        if d_chk_pos.numel() % 3:
+            raise ValueError("check positions must hold 3 values per point")
+        n = d_chk_pos.numel() // 3
+        self._chk(lib().kifmm_direct_device_noalloc(
+            self._h, _dev(d_src_den, self.n_src * self.src_dof, "densities"), n,
+            _dev(d_chk_pos, 3 * n, "check positions"), _dev(d_chk_val, n * self.trg_dof, "check values"),
+            _stream(stream)))
 
     def check(self, src_den, trg_val, numchk, seed=0):
         d = _f64(src_den)

@@ -145,6 +145,15 @@ int kifmm_field_checkpoint_device(kifmm_handle h, const double *d_src_den, const
 int kifmm_direct_device(kifmm_handle h, const double *d_src_den, int nchk, const double *d_chk_pos,
                         double *d_chk_val, void *stream);
 
+/* kifmm_direct_device that allocates nothing, for callers that cannot call
+ * hipMalloc (code under graph capture, serving loops, harnesses that forbid
+ * it).  It has no counterpart in the reference.  The partial sums go to a
+ * fixed scratch array the library holds on each GPU (2^21 doubles, 16 MiB,
+ * kept until the process ends); nchk * trgDOF above 2^21 is rejected with
+ * KIFMM_ERR_ARG.  Calls that overlap on one GPU share that scratch and race. */
+int kifmm_direct_device_noalloc(kifmm_handle h, const double *d_src_den, int nchk, const double *d_chk_pos,
+                                double *d_chk_val, void *stream);
+
 /* FMM3d_MPI::check (fmm3d_check_mpi.cpp:34-173): relative 2-norm error of
  * trg_val against the direct sum at numchk targets sampled with `seed`
  * (all targets when numchk >= n_trg).  Host pointers. */
