@@ -373,6 +373,45 @@ int kifmm_set_tree_builder(kifmm_handle h, int on_device) {
   return KIFMM_OK;
 }
 
+int kifmm_set_translation(kifmm_handle h, int terms) {
+  if (!h) return set_err(nullptr, KIFMM_ERR_ARG, "null handle");
+  return h->fmm.setTranslationTerms(terms);
+}
+
+int kifmm_autotune(kifmm_handle h, const double* d_src_den, double* d_trg_val, int nrhs, void* stream) {
+  if (!h) return set_err(nullptr, KIFMM_ERR_ARG, "null handle");
+  return h->fmm.autotune(d_src_den, d_trg_val, nrhs, static_cast<hipStream_t>(stream));
+}
+
+int kifmm_get_tuning(kifmm_handle h, int* knobs, int n) {
+  if (!h || !knobs) return set_err(h, KIFMM_ERR_ARG, "null argument");
+  const kifmm::Tuning& t = h->fmm.tuning();
+  const int v[KIFMM_NTUNE] = {t.near_mode, t.near_stage, t.trans, t.vec_hadamard};
+  for (int i = 0; i < n && i < KIFMM_NTUNE; i++) knobs[i] = v[i];
+  return KIFMM_OK;
+}
+
+int kifmm_set_tuning(kifmm_handle h, const int* knobs, int n) {
+  if (!h || !knobs || n < KIFMM_NTUNE) return set_err(h, KIFMM_ERR_ARG, "need KIFMM_NTUNE knob values");
+  kifmm::Tuning t;
+  t.near_mode = knobs[0];
+  t.near_stage = knobs[1];
+  t.trans = knobs[2];
+  t.vec_hadamard = knobs[3];
+  return h->fmm.apply_tuning(t);
+}
+
+int kifmm_tuning_report(kifmm_handle h, double* rows, int maxrows) {
+  if (!h) return set_err(nullptr, KIFMM_ERR_ARG, "null handle");
+  const auto& r = h->fmm.tuneReport();
+  for (size_t i = 0; rows && i < r.size() && int(i) < maxrows; i++) {
+    rows[3 * i] = r[i].knob;
+    rows[3 * i + 1] = r[i].value;
+    rows[3 * i + 2] = r[i].ms;
+  }
+  return int(r.size());
+}
+
 int kifmm_setup_times(kifmm_handle h, double* ms, int n) {
   if (!h || !ms) return set_err(h, KIFMM_ERR_ARG, "null argument");
   const auto& s = h->fmm.setupMs();

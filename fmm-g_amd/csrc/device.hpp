@@ -55,6 +55,12 @@ struct NearArgs {
   // right-hand sides r0 .. r0 + nr - 1.
   int nrhs = 1, r0 = 0, nr = 1;
   int64_t nstride = 0, tstride = 0;
+  // Mixed near kernels: chunk c of the launch is chunks[order[c]] (order null:
+  // chunks[c]); with queue set, a persistent grid of workgroups takes chunk
+  // indices from the global counter *queue (zeroed by the launcher) instead
+  // of one workgroup per chunk.
+  const int* order = nullptr;
+  int* queue = nullptr;
 };
 
 struct SurfArgs {
@@ -116,6 +122,10 @@ struct VlistArgs {
   // consecutive virtual groups share their translation spectra.
   int nrhs = 1;
   int64_t nstride = 0;
+  // mixed Hadamard of the vector kernels (sd, td > 1): 0 one wave per group
+  // reading the translation spectra per pair, 1 and 2 the halo sweep over tiles
+  // of 1 or 2 groups
+  int had_form = 0;
 };
 
 // Adjoint near field (source-centric): chunk = <= kNearChunk sources of a
@@ -174,6 +184,10 @@ hipError_t launch_surface_mixed(int kt, const SurfArgs& a, const KParams<double>
 hipError_t launch_hadamard_mixed(const VlistArgs& v, const float2* dspec, const float2* m2l, float2* espec,
                                  hipStream_t s);
 hipError_t launch_gemm_fma(const GemmArgs& g, hipStream_t s);
+// Translation GEMM as split-BF16 products on the matrix cores (kernels_bf16.hip):
+// a3 holds A's three BF16 parts [3][nz][Mp][Kp] (row-major, Mp a multiple of
+// 16 and Kp of 32, zero padded), terms 3 (BF16x3) or 6 (BF16x6); no split K.
+hipError_t launch_gemm_bf16s(const GemmArgs& g, const void* a3, int Mp, int Kp, int terms, hipStream_t s);
 hipError_t launch_fft_fwd32(const VlistArgs& v, float2* dspec, hipStream_t s);
 hipError_t launch_fft_inv32(const VlistArgs& v, const float2* espec, hipStream_t s);
 

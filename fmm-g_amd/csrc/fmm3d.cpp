@@ -115,6 +115,36 @@ int FMM3d::setup() {
 
 int FMM3d::setupHost() { return setup_prepare(false); }
 
+int FMM3d::setTranslationTerms(int terms) {
+  if (terms != 0 && terms != 3 && terms != 6) return fail(-2, "translation terms must be 0 (FP64), 3 or 6 (split BF16)");
+  if (terms != trans_terms_) {
+    trans_terms_ = terms;
+    tune_.trans = terms == 6 ? 6 : 0;
+    drop_graph();
+    setup_done_ = false;  // the step tables differ: setup() again
+    host_ready_ = false;
+  }
+  return 0;
+}
+
+// x = p0 + p1 + p2 + O(2^-24 |x|): each part the BF16 (round to nearest even)
+// of what the previous parts leave, via FP32.
+void split_bf16x3(double x, uint16_t* p) {
+  double r = x;
+  for (int q = 0; q < 3; q++) {
+    const float f = float(r);
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) { p[q] = uint16_t(u >> 16); r = 0.0; continue; }  // inf / nan
+    u += 0x7fffu + ((u >> 16) & 1u);
+    p[q] = uint16_t(u >> 16);
+    const uint32_t back = uint32_t(p[q]) << 16;
+    float b;
+    std::memcpy(&b, &back, 4);
+    r -= double(b);
+  }
+}
+
 int FMM3d::setup_prepare(bool device_tree) {
   using clk = std::chrono::steady_clock;
   auto ms_since = [](clk::time_point a) { return std::chrono::duration<double, std::milli>(clk::now() - a).count(); };
@@ -181,6 +211,94 @@ int FMM3d::setup_prepare(bool device_tree) {
   return 0;
 }
 
+// A near-field chunk of leaf B: targets [c, c + k) of the leaf, items
+// [ibeg, iend); own / unchk from the bounding boxes (see NearChunk).
+NearChunk FMM3d::make_near_chunk(int B, int c, int k, int ibeg, int iend) const {
+  const Tree& t = tree_;
+  NearChunk ch{B, t.trg_beg[B] + c, k, ibeg, iend};
+  double tb[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
+  for (int j = 0; j < ch.tnum; j++) {
+    const int64_t o = t.trg_perm[size_t(ch.tbeg) + j];
+    for (int d = 0; d < 3; d++) {
+      tb[d] = std::min(tb[d], tpos_[3 * o + d]);
+      tb[3 + d] = std::max(tb[3 + d], tpos_[3 * o + d]);
+    }
+  }
+  bool sep = true;
+  for (int i = ibeg; i < iend && sep; i++) {
+    const int code = near_items_h_[size_t(i)], A = code >> 2;
+    if ((code & 3) != ITEM_EXACT) continue;  // surfaces of non-adjacent nodes or the DE surface
+    if (A == B && ch.own < 0) { ch.own = i - ibeg; continue; }
+    const double* bx = &sbox_[size_t(A) * 6];
+    double d2 = 0;
+    for (int d = 0; d < 3; d++) {
+      const double g = std::max(0.0, std::max(bx[d] - tb[3 + d], tb[d] - bx[3 + d]));
+      d2 += g * g;
+    }
+    if (d2 < 1e-14) sep = false;  // within 1e-7: keep the cut on every pair
+  }
+  ch.unchk = sep ? 1 : 0;
+  return ch;
+}
+
+const double FMM3d::kStageWeights[3] = {0.04, 0.15, 0.30};
+
+// Mixed path: the near kernels put S = min(16, 128 / k) copies of a chunk's k
+// targets on the 128 lanes, so a leaf of 70 targets leaves 58 lanes idle.
+// Leaves are cut into chunks of sizes that fill the lanes: cost of a chunk =
+// (active waves) / S sweeps of its sources, plus the staging weight per wave
+// for staging them again; the cut minimises the sum (dynamic programming over
+// sizes).  The chunks are also ordered by that cost, largest first (the order
+// of the work queue).
+void FMM3d::build_mixed_chunks() {
+  const Tree& t = tree_;
+  const double wstage = kStageWeights[tune_.near_stage];
+  int max_tn = 0;
+  for (int B : near_leaves_) max_tn = std::max(max_tn, t.trg_num[B]);
+  std::vector<double> split_cost(size_t(max_tn) + 1, 0.0);
+  std::vector<int> split_pick(size_t(max_tn) + 1, 0);
+  for (int n = 1; n <= max_tn; n++) {
+    split_cost[n] = 1e300;
+    for (int k = 1; k <= std::min(n, kNearChunk); k++) {
+      const int S = std::min(16, kNearChunk / k);
+      const double c = double((S * k + 63) / 64) / S + 2 * wstage + split_cost[n - k];
+      if (c < split_cost[n] - 1e-12) { split_cost[n] = c; split_pick[n] = k; }
+    }
+  }
+  const int NL = int(near_leaves_.size());
+  std::vector<int64_t> moff(size_t(NL) + 1, 0);
+  for (int l = 0; l < NL; l++) {
+    const int tn = t.trg_num[near_leaves_[l]];
+    int nm = 0;
+    for (int c = 0; c < tn; c += split_pick[tn - c]) nm++;
+    moff[size_t(l) + 1] = moff[size_t(l)] + nm;
+  }
+  near_chunks_mx_h_.assign(size_t(moff[size_t(NL)]), NearChunk{});
+  std::vector<double> cost(near_chunks_mx_h_.size(), 0.0);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int l = 0; l < NL; l++) {
+    const int B = near_leaves_[l], tn = t.trg_num[B];
+    const int ibeg = int(near_ioff_[size_t(l)]), iend = int(near_ioff_[size_t(l) + 1]);
+    double src = 0;  // source points over the leaf's items
+    for (int i = ibeg; i < iend; i++) {
+      const int code = near_items_h_[size_t(i)], A = code >> 2, kind = code & 3;
+      src += kind == ITEM_EXACT ? t.src_num[A] : ops_->n_sam[kind == ITEM_UE ? UE : DE];
+    }
+    int64_t mi = moff[size_t(l)];
+    for (int c = 0; c < tn;) {
+      const int k = split_pick[tn - c];
+      const int S = std::min(16, kNearChunk / k);
+      cost[size_t(mi)] = double((S * k + 63) / 64) / S * src;
+      near_chunks_mx_h_[size_t(mi++)] = make_near_chunk(B, c, k, ibeg, iend);
+      c += k;
+    }
+  }
+  near_order_mx_h_.resize(near_chunks_mx_h_.size());
+  for (size_t i = 0; i < near_order_mx_h_.size(); i++) near_order_mx_h_[i] = int(i);
+  std::stable_sort(near_order_mx_h_.begin(), near_order_mx_h_.end(),
+                   [&](int a, int b) { return cost[size_t(a)] > cost[size_t(b)]; });
+}
+
 int FMM3d::build_tasks() {
   const Tree& t = tree_;
   const Operators& op = *ops_;
@@ -190,27 +308,11 @@ int FMM3d::build_tasks() {
 
   // --- near field: U, W (S2T or M2T), X (S2T fallback), L2T ---
   near_chunks_h_.clear(); near_chunks_mx_h_.clear(); near_items_h_.clear();
-  // Mixed path: the near kernels put S = min(16, 128 / k) copies of a chunk's k
-  // targets on the 128 lanes, so a leaf of 70 targets leaves 58 lanes idle.
-  // Leaves are cut into chunks of sizes that fill the lanes: cost of a chunk =
-  // (active waves) / S sweeps of its sources, plus 0.15 per wave for staging
-  // them again; the cut minimises the sum (dynamic programming over sizes).
-  int max_tn = 0;
-  for (int B = 0; B < N; B++)
-    if ((t.tag[B] & TAG_EVTR) && t.leaf(B)) max_tn = std::max(max_tn, t.trg_num[B]);
-  std::vector<double> split_cost(size_t(max_tn) + 1, 0.0);
-  std::vector<int> split_pick(size_t(max_tn) + 1, 0);
-  for (int n = 1; n <= max_tn; n++) {
-    split_cost[n] = 1e300;
-    for (int k = 1; k <= std::min(n, kNearChunk); k++) {
-      const int S = std::min(16, kNearChunk / k);
-      const double c = double((S * k + 63) / 64) / S + 2 * 0.15 + split_cost[n - k];
-      if (c < split_cost[n] - 1e-12) { split_cost[n] = c; split_pick[n] = k; }
-    }
-  }
   // bounding boxes of every leaf's sources (the near kernel of the mixed path
   // applies the 1e-8 pair cut only where a source can come that close)
-  std::vector<double> sbox(size_t(N) * 6);
+  sbox_.assign(size_t(N) * 6, 0.0);
+  std::vector<double>& sbox = sbox_;
+#pragma omp parallel for schedule(dynamic, 256)
   for (int A = 0; A < N; A++) {
     double* bx = &sbox[size_t(A) * 6];
     for (int d = 0; d < 3; d++) { bx[d] = 1e300; bx[3 + d] = -1e300; }
@@ -223,72 +325,77 @@ int FMM3d::build_tasks() {
       }
     }
   }
-  for (int B = 0; B < N; B++) {
-    if (!(t.tag[B] & TAG_EVTR) || !t.leaf(B)) continue;
-    const int ibeg = int(near_items_h_.size());
+  // Evaluator leaves in node order; their items and chunks are counted first,
+  // then filled in parallel at their prefix offsets (the same order as a
+  // sequential pass over the leaves).
+  std::vector<int>& near_leaves = near_leaves_;
+  near_leaves.clear();
+  for (int B = 0; B < N; B++)
+    if ((t.tag[B] & TAG_EVTR) && t.leaf(B)) near_leaves.push_back(B);
+  const int NL = int(near_leaves.size());
+  near_ioff_.assign(size_t(NL) + 1, 0);
+  std::vector<int64_t>& ioff = near_ioff_;
+  std::vector<int64_t> coff(size_t(NL) + 1, 0);
+  auto x_fallback = [&](int B) { return t.trg_num[B] * TD < szDC; };  // :1249
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int l = 0; l < NL; l++) {
+    const int B = near_leaves[l], tn = t.trg_num[B];
+    int64_t ni = (t.off[LIST_U][B + 1] - t.off[LIST_U][B]) + (t.off[LIST_W][B + 1] - t.off[LIST_W][B]);
+    if (x_fallback(B)) ni += t.off[LIST_X][B + 1] - t.off[LIST_X][B];
+    if (t.depth[B] >= 2) ni++;
+    ioff[size_t(l) + 1] = ni;
+    coff[size_t(l) + 1] = (tn + kNearChunk - 1) / kNearChunk;
+  }
+  for (int l = 0; l < NL; l++) {
+    ioff[size_t(l) + 1] += ioff[size_t(l)];
+    coff[size_t(l) + 1] += coff[size_t(l)];
+  }
+  if (ioff[size_t(NL)] > INT32_MAX) return fail(-2, "too many near-field items for 32-bit offsets");
+  near_items_h_.assign(size_t(ioff[size_t(NL)]), 0);
+  near_chunks_h_.assign(size_t(coff[size_t(NL)]), NearChunk{});
+  double near_exact = 0, near_all = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : near_exact, near_all)
+  for (int l = 0; l < NL; l++) {
+    const int B = near_leaves[l];
+    const int ibeg = int(ioff[size_t(l)]);
+    int* it = near_items_h_.data();
+    int ni = ibeg;
     const int tn = t.trg_num[B];
     double exact = 0, surf = 0;
     for (int64_t q = t.off[LIST_U][B]; q < t.off[LIST_U][B + 1]; q++) {
       const int A = t.lst[LIST_U][q];
-      near_items_h_.push_back(A << 2 | ITEM_EXACT);
+      it[ni++] = A << 2 | ITEM_EXACT;
       exact += t.src_num[A];
     }
     for (int64_t q = t.off[LIST_W][B]; q < t.off[LIST_W][B + 1]; q++) {  // :1146-1165
       const int A = t.lst[LIST_W][q];
       if (t.src_beg[A] >= 0 && t.src_num[A] * SD < szUE) {
-        near_items_h_.push_back(A << 2 | ITEM_EXACT);
+        it[ni++] = A << 2 | ITEM_EXACT;
         exact += t.src_num[A];
       } else {
-        near_items_h_.push_back(A << 2 | ITEM_UE);
+        it[ni++] = A << 2 | ITEM_UE;
         surf += op.n_sam[UE];
       }
     }
-    if (tn * TD < szDC)  // :1249
+    if (x_fallback(B))
       for (int64_t q = t.off[LIST_X][B]; q < t.off[LIST_X][B + 1]; q++) {
         const int A = t.lst[LIST_X][q];
-        near_items_h_.push_back(A << 2 | ITEM_EXACT);
+        it[ni++] = A << 2 | ITEM_EXACT;
         exact += t.src_num[A];
       }
     if (t.depth[B] >= 2) {  // DE is zero above depth 2 (:1378)
-      near_items_h_.push_back(B << 2 | ITEM_DE);
+      it[ni++] = B << 2 | ITEM_DE;
       surf += op.n_sam[DE];
     }
-    const int iend = int(near_items_h_.size());
-    auto make_chunk = [&](int c, int k) {
-      NearChunk ch{B, t.trg_beg[B] + c, k, ibeg, iend};
-      double tb[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
-      for (int j = 0; j < ch.tnum; j++) {
-        const int64_t o = t.trg_perm[size_t(ch.tbeg) + j];
-        for (int d = 0; d < 3; d++) {
-          tb[d] = std::min(tb[d], tpos_[3 * o + d]);
-          tb[3 + d] = std::max(tb[3 + d], tpos_[3 * o + d]);
-        }
-      }
-      bool sep = true;
-      for (int i = ibeg; i < iend && sep; i++) {
-        const int code = near_items_h_[i], A = code >> 2;
-        if ((code & 3) != ITEM_EXACT) continue;  // surfaces of non-adjacent nodes or the DE surface
-        if (A == B && ch.own < 0) { ch.own = i - ibeg; continue; }
-        const double* bx = &sbox[size_t(A) * 6];
-        double d2 = 0;
-        for (int d = 0; d < 3; d++) {
-          const double g = std::max(0.0, std::max(bx[d] - tb[3 + d], tb[d] - bx[3 + d]));
-          d2 += g * g;
-        }
-        if (d2 < 1e-14) sep = false;  // within 1e-7: keep the cut on every pair
-      }
-      ch.unchk = sep ? 1 : 0;
-      return ch;
-    };
-    for (int c = 0; c < tn; c += kNearChunk) near_chunks_h_.push_back(make_chunk(c, std::min(kNearChunk, tn - c)));
-    for (int c = 0; c < tn;) {
-      const int k = split_pick[tn - c];
-      near_chunks_mx_h_.push_back(make_chunk(c, k));
-      c += k;
-    }
-    work_.near_exact_pairs += double(tn) * exact;
-    work_.near_pairs += double(tn) * (exact + surf);
+    int64_t ci = coff[size_t(l)];
+    for (int c = 0; c < tn; c += kNearChunk)
+      near_chunks_h_[size_t(ci++)] = make_near_chunk(B, c, std::min(kNearChunk, tn - c), ibeg, ni);
+    near_exact += double(tn) * exact;
+    near_all += double(tn) * (exact + surf);
   }
+  work_.near_exact_pairs = near_exact;
+  work_.near_pairs = near_all;
+  build_mixed_chunks();
 
   // --- S2M (:557-577) and X-list S2L (:1241-1257) ---
   s2m_tasks_h_.clear(); s2l_tasks_h_.clear(); surf_items_h_.clear(); s2m_leaf_h_.clear();
@@ -330,7 +437,7 @@ int FMM3d::build_tasks() {
   const int rl = t.root_level;
   std::vector<std::vector<int>> by_depth(t.max_depth + 1);
   for (int i = 0; i < N; i++) by_depth[t.depth[i]].push_back(i);
-  const bool fuse_up = precision_ == PREC_MIXED;
+  const bool fuse_up = precision_ == PREC_MIXED && trans_terms_ == 0;
   for (int d = t.max_depth; d >= 1; d--) {  // :557-595, children before parents
     std::vector<int> mb, mc, pb, lb;
     for (int i : by_depth[d]) {
@@ -447,6 +554,8 @@ int FMM3d::build_vgroups(const std::vector<int>& order, const std::vector<int64_
   const int ngrp = int(grp.size());
   // The halo-sweep Hadamard relies on V(tau) = {occupied halo slots not adjacent
   // to tau} for every target child tau of a group; check it on the lists.
+  int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
   for (int g = 0; g < ngrp; g++)
     for (int tau = 0; tau < 8; tau++) {
       if (vg.tgt[size_t(g) * 8 + tau] < 0) continue;
@@ -456,14 +565,14 @@ int FMM3d::build_vgroups(const std::vector<int>& order, const std::vector<int64_
         bool adj = true;
         for (int d = 0; d < 3; d++) adj = adj && std::abs(rc[d] - tc[d]) <= 1;
         const bool in_v = (vg.mask[size_t(g) * 32 + tau * 4 + (r >> 6)] >> (r & 63)) & 1;
-        if (in_v != (!adj && vg.halo[size_t(g) * 216 + r] >= 0))
-          return fail(-6, "V list differs from the non-adjacent halo of its sibling group");
+        if (in_v != (!adj && vg.halo[size_t(g) * 216 + r] >= 0)) bad = 1;
       }
     }
+  if (bad) return fail(-6, "V list differs from the non-adjacent halo of its sibling group");
   // per group: halo slots ordered by node index (the V lists' order)
   vg.hord.assign(size_t(ngrp) * 216, -1);
-  for (auto& pg : grp) {
-    const int g = pg.second;
+#pragma omp parallel for schedule(static)
+  for (int g = 0; g < ngrp; g++) {
     std::vector<std::pair<int, int>> nodes;
     for (int r = 0; r < 216; r++) {
       int sl = vg.halo[size_t(g) * 216 + r];
@@ -475,6 +584,7 @@ int FMM3d::build_vgroups(const std::vector<int>& order, const std::vector<int64_
   vg.psrc.assign(size_t(ngrp) * kPairStride, -1);
   vg.pslot.assign(size_t(ngrp) * kPairStride, 0);
   vg.pcnt.assign(ngrp, 0);
+#pragma omp parallel for schedule(static)
   for (int g = 0; g < ngrp; g++) {
     int k = 0;
     for (; k < 216 && vg.hord[size_t(g) * 216 + k] >= 0; k++) {
@@ -547,6 +657,8 @@ int FMM3d::upload_state() {
   // tasks
   KIFMM_UP(near_chunks_d_, near_chunks_h_, "upload near chunks");
   KIFMM_UP(near_chunks_mx_d_, near_chunks_mx_h_, "upload near chunks");
+  KIFMM_UP(near_order_mx_d_, near_order_mx_h_, "upload near chunk order");
+  if (!queue_d_.bytes() && (rc = check(queue_d_.alloc(256), "alloc work-queue counter"))) return rc;
   KIFMM_UP(near_items_d_, near_items_h_, "upload near items");
   KIFMM_UP(s2m_tasks_d_, s2m_tasks_h_, "upload S2M tasks");
   KIFMM_UP(s2m_leaf_d_, s2m_leaf_h_, "upload S2M tasks");
@@ -591,6 +703,32 @@ int FMM3d::upload_state() {
     wup_d_.release();
   }
   KIFMM_UP(l2l_d_, op.l2l, "upload L2L");
+  if (precision_ == PREC_MIXED && trans_terms_ > 0) {
+    // the translation tables as three BF16 parts each, row-major and padded
+    auto split_table = [&](const std::vector<double>& a, int nz, int M, int K, int& Mp, int& Kp,
+                           DeviceBuffer& out) -> int {
+      Mp = (M + 15) / 16 * 16;
+      Kp = (K + 31) / 32 * 32;
+      const size_t per = size_t(nz) * Mp * Kp;
+      std::vector<uint16_t> parts(3 * per, 0);
+#pragma omp parallel for
+      for (int zm = 0; zm < nz * Mp; zm++) {
+        const int zz = zm / Mp, m = zm % Mp;
+        for (int k = 0; k < Kp; k++) {
+          const double v = m < M && k < K ? a[size_t(zz) * M * K + size_t(k) * M + m] : 0.0;
+          uint16_t p[3];
+          split_bf16x3(v, p);
+          for (int q = 0; q < 3; q++) parts[size_t(q) * per + size_t(zm) * Kp + k] = p[q];
+        }
+      }
+      return check(out.upload(parts.data(), parts.size() * sizeof(uint16_t)), "upload BF16 translation table");
+    };
+    if ((rc = split_table(op.m2m, 1, op.dat(UC), 8 * op.dat(UE), m2m_mp_, m2m_kp_, m2m_bf_d_))) return rc;
+    if ((rc = split_table(op.l2l, 8, op.dat(DC), op.dat(DE), l2l_mp_, l2l_kp_, l2l_bf_d_))) return rc;
+  } else {
+    m2m_bf_d_.release();
+    l2l_bf_d_.release();
+  }
   if (precision_ == PREC_MIXED) {
     // split-K workspace: the same split rule as the evaluate over every step
     size_t need = 1;
@@ -668,6 +806,7 @@ void FMM3d::vlist_args(VlistArgs& v, const VGroups& vg) const {
   v.psrc = vg.psrc_d.as<int>(); v.pslot = vg.pslot_d.as<int>(); v.pcnt = vg.pcnt_d.as<int>();
   v.mask = vg.mask_d.as<uint64_t>(); v.tgt = vg.tgt_d.as<int>();
   v.m2l = m2l_d_.as<double2>(); v.dspec = dspec_d_.as<double2>(); v.espec = espec_d_.as<double2>();
+  v.had_form = tune_.vec_hadamard;
 }
 
 // timed: record the Hadamard's start and end events (forward pass with timing on).
@@ -979,6 +1118,15 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
       if (st.rs_off >= 0) { g.rscale = scales_d_.as<double>() + st.rs_off; g.rsd = op.sd; }
       else { g.rscale = ones_d_.as<double>(); g.rsd = 1; }
       g.ncol = st.ncol; g.accumulate = st.accumulate;
+      if (mixed && trans_terms_ > 0 && (st.mat == 0 || st.mat == 3)) {
+        // translations as split-BF16 products (the pseudo-inverses stay FP64)
+        const bool m2m = st.mat == 0;
+        int r = check(launch_gemm_bf16s(g, m2m ? m2m_bf_d_.as<void>() : l2l_bf_d_.as<void>(), m2m ? m2m_mp_ : l2l_mp_,
+                                        m2m ? m2m_kp_ : l2l_kp_, trans_terms_, s),
+                      "BF16 translation GEMM");
+        if (r) return r;
+        continue;
+      }
       // mixed path: launches with very few workgroups split K into chunks of
       // whole 16-deep slabs; the chunks' partial products go to the workspace
       // and are summed in chunk order into the output
@@ -1036,7 +1184,12 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
   mark(PH_NEAR);
   NearArgs na{};
   na.chunks = near_chunks_d_.as<NearChunk>(); na.nchunks = int(near_chunks_h_.size());
-  if (mixed && !grad) { na.chunks = near_chunks_mx_d_.as<NearChunk>(); na.nchunks = int(near_chunks_mx_h_.size()); }
+  if (mixed && !grad) {
+    na.chunks = near_chunks_mx_d_.as<NearChunk>();
+    na.nchunks = int(near_chunks_mx_h_.size());
+    if (tune_.near_mode >= 1) na.order = near_order_mx_d_.as<int>();
+    if (tune_.near_mode == 2) na.queue = queue_d_.as<int>();
+  }
   na.items = near_items_d_.as<int>();
   na.spos = spos_d_.as<double>(); na.snor = snor_d_.as<double>(); na.sden = sden_d_.as<double>();
   na.src_beg = sbeg_d_.as<int>(); na.src_num = snum_d_.as<int>();
@@ -1078,6 +1231,105 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
     phase_ms_[PH_HADAMARD] = ms;
   }
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: each launch-shape choice of the mixed path is timed on this
+// handle's own tree and inputs (phase times of warm evaluates, the best of
+// two), one choice at a time, and the fastest kept.
+// ---------------------------------------------------------------------------
+int FMM3d::retask() {
+  drop_graph();
+  int rc = build_tasks();
+  if (rc) return rc;
+  rhs_steps_n_ = 1;
+  up_steps_r_.clear(); dn_steps_r_.clear(); lvl_idx_r_h_.clear();
+  rhs_cap_ = 1;
+  return upload_state();
+}
+
+int FMM3d::apply_tuning(const Tuning& t) {
+  if (t.near_mode < 0 || t.near_mode > 2 || t.near_stage < 0 || t.near_stage > 2 || (t.trans != 0 && t.trans != 6) ||
+      t.vec_hadamard < 0 || t.vec_hadamard > 2)
+    return fail(-2, "tuning values out of range");
+  if (setup_done_ && t.vec_hadamard != tune_.vec_hadamard) drop_graph();
+  if (!setup_done_) { tune_ = t; trans_terms_ = t.trans; return 0; }
+  const bool new_tasks = t.trans != trans_terms_;
+  const bool new_chunks = t.near_stage != tune_.near_stage;
+  tune_ = t;
+  trans_terms_ = t.trans;
+  int rc = 0;
+  if (new_tasks) {
+    if ((rc = retask())) { setup_done_ = false; return rc; }
+  } else if (new_chunks) {
+    drop_graph();
+    build_mixed_chunks();
+    if ((rc = check(near_chunks_mx_d_.upload(near_chunks_mx_h_.data(), sizeof(NearChunk) * near_chunks_mx_h_.size()),
+                    "upload near chunks")) ||
+        (rc = check(near_order_mx_d_.upload(near_order_mx_h_.data(), sizeof(int) * near_order_mx_h_.size()),
+                    "upload near chunk order")))
+      return rc;
+  }
+  return 0;
+}
+
+int FMM3d::autotune(const double* d_src_den, double* d_trg_val, int nrhs, hipStream_t s) {
+  if (!setup_done_) return fail(-3, "autotune called before setup");
+  if (nrhs < 1) return fail(-2, "nrhs must be >= 1");
+  tune_report_.clear();
+  if (precision_ != PREC_MIXED) return 0;  // the FP64 path keeps its fixed summation order
+  const bool was_timing = timing_;
+  timing_ = true;
+  auto run = [&](double* ms_out) -> int {
+    double best[PH_COUNT];
+    for (int p = 0; p < PH_COUNT; p++) best[p] = 1e300;
+    for (int rep = 0; rep < 3; rep++) {  // one warm-up, then the best of two
+      const int rc = nrhs == 1 ? evaluate(d_src_den, d_trg_val, s) : evaluateMulti(d_src_den, d_trg_val, nrhs, s);
+      if (rc) return rc;
+      if (rep == 0) continue;
+      for (int p = 0; p < PH_COUNT; p++) best[p] = std::min(best[p], double(phase_ms_[size_t(p)]));
+    }
+    for (int p = 0; p < PH_COUNT; p++) ms_out[p] = best[p];
+    return 0;
+  };
+  auto total = [](const double* ms) {
+    double t = 0;
+    for (int p = 0; p < PH_COUNT; p++) t += ms[p];
+    return t;
+  };
+  struct Knob { int id; std::vector<int> values; };
+  std::vector<Knob> knobs = {{2, {0, 6}}, {1, {0, 1, 2}}, {0, {0, 1, 2}}};
+  if (ops_->sd > 1 || ops_->td > 1) knobs.push_back({3, {0, 1, 2}});
+  int rc = 0;
+  for (const Knob& k : knobs) {
+    Tuning base = tune_;
+    int best_v = -1;
+    double best_t = 1e300;
+    for (int v : k.values) {
+      Tuning t = base;
+      if (k.id == 0) t.near_mode = v;
+      else if (k.id == 1) t.near_stage = v;
+      else if (k.id == 2) t.trans = v;
+      else t.vec_hadamard = v;
+      if ((rc = apply_tuning(t))) break;
+      double ms[PH_COUNT];
+      if ((rc = run(ms))) break;
+      // the translations touch the up and down passes, the near-field choices
+      // the near-field phase, the Hadamard form the V-list phase
+      const double tm = k.id == 2 ? total(ms) : k.id == 3 ? ms[PH_VLIST] : ms[PH_NEAR];
+      tune_report_.push_back({k.id, v, tm});
+      if (tm < best_t) { best_t = tm; best_v = v; }
+    }
+    if (rc) break;
+    Tuning t = base;
+    if (k.id == 0) t.near_mode = best_v;
+    else if (k.id == 1) t.near_stage = best_v;
+    else if (k.id == 2) t.trans = best_v;
+    else t.vec_hadamard = best_v;
+    if ((rc = apply_tuning(t))) break;
+  }
+  timing_ = was_timing;
+  return rc;
 }
 
 int FMM3d::direct(const double* d_src_den, int nchk, const double* d_chk_pos, double* d_chk_val,

@@ -54,14 +54,29 @@ struct VGroups {
   int ngrp() const { return int(tgt.size() / 8); }
 };
 
+// x as three BF16 parts (bit patterns), each the rounding of what the previous
+// ones leave: x = p0 + p1 + p2 to about 2^-24 |x|.
+void split_bf16x3(double x, uint16_t* p);
+
 // Split K of a mixed-path level GEMM launch that would have too few workgroups.
 void split_k(GemmArgs& g);
 // The split of a launch over nrhs right-hand sides is the one-RHS launch's
 // (same K chunks per column), so every column sums in the one-RHS order.
 void split_k_rhs(GemmArgs& g, int nrhs);
 
+// Launch shapes the autotuner chooses (FMM3d::autotune), per handle.
+struct Tuning {
+  int near_mode = 1;   // mixed near field: 0 one workgroup per chunk in leaf order, 1 the same in cost
+                       // order (largest first), 2 a persistent grid over a global work queue in cost order
+  int near_stage = 1;  // staging weight of the chunk-split cost model: FMM3d::kStageWeights[near_stage]
+  int trans = 0;       // mixed translations: 0 FP64 matrix cores (fused M2M), 6 split-BF16x6
+  int vec_hadamard = 0;  // mixed Hadamard of vector kernels: 0 per pair, 1 / 2 halo sweep, 1 / 2 groups a tile
+};
+constexpr int kNumTuning = 4;
+
 class FMM3d {
  public:
+  static const double kStageWeights[3];
   FMM3d() = default;
   ~FMM3d();
 
@@ -95,6 +110,21 @@ class FMM3d {
   // both give the same tree.
   int setup();
   void setTreeOnDevice(bool on) { tree_on_device_ = on; }
+  // Mixed path translations (M2M, L2L): 0 = FP64 matrix cores with M2M fused
+  // into the upward pseudo-inverse (default), 3 = split-BF16x3, 6 = split-BF16x6
+  // products on the BF16 matrix cores.  Takes effect at the next setup().
+  int setTranslationTerms(int terms);
+  int translationTerms() const { return trans_terms_; }
+  // Autotuner: times the mixed path's launch-shape choices (Tuning) on this
+  // handle's tree with the caller's densities (nrhs right-hand sides), one
+  // choice at a time, and keeps the fastest; d_trg_val receives the values of
+  // the last trial.  No-op for the FP64 path.
+  int autotune(const double* d_src_den, double* d_trg_val, int nrhs, hipStream_t stream);
+  const Tuning& tuning() const { return tune_; }
+  // Applies given choices (e.g. a previous autotune's); re-uploads what changes.
+  int apply_tuning(const Tuning& t);
+  struct TuneTrial { int knob, value; double ms; };
+  const std::vector<TuneTrial>& tuneReport() const { return tune_report_; }
   // Wall times of the last setup, milliseconds: tree (total), of which device
   // build and copy-back (device builder only), operators, task tables, upload.
   enum SetupPart { SU_TREE = 0, SU_TREE_DEVICE, SU_TREE_COPY, SU_OPS, SU_TASKS, SU_UPLOAD, SU_COUNT };
@@ -171,6 +201,9 @@ class FMM3d {
   int prepare_rhs_steps(int nrhs);
   void drop_graph();
   int build_tasks();
+  int retask();  // task tables and device state again (after a change of translations)
+  NearChunk make_near_chunk(int B, int c, int k, int ibeg, int iend) const;
+  void build_mixed_chunks();
   int upload_state();
   int build_vgroups(const std::vector<int>& order, const std::vector<int64_t>& off, const std::vector<int>& lst,
                     bool need_evtr, VGroups& vg);
@@ -187,6 +220,11 @@ class FMM3d {
   std::vector<double> spos_, snor_, tpos_;
   std::vector<int64_t> prob_ns_, prob_nt_;  // forest: points per problem (empty: one problem)
   bool setup_done_ = false, host_ready_ = false, timing_ = false, tree_on_device_ = true;
+  int trans_terms_ = 0;
+  Tuning tune_;
+  std::vector<TuneTrial> tune_report_;
+  DeviceBuffer m2m_bf_d_, l2l_bf_d_;  // split-BF16 translation tables [3][nz][Mp][Kp]
+  int m2m_mp_ = 0, m2m_kp_ = 0, l2l_mp_ = 0, l2l_kp_ = 0;
   std::array<double, SU_COUNT> setup_ms_{};
   std::string err_;
   Tree tree_;
@@ -222,6 +260,11 @@ class FMM3d {
   std::vector<double> scales_h_;
   std::vector<NearChunk> near_chunks_h_;
   std::vector<NearChunk> near_chunks_mx_h_;  // mixed near field: leaves split for full lanes
+  std::vector<int> near_order_mx_h_;         // the mixed chunks by estimated cost, largest first
+  std::vector<int> near_leaves_;             // evaluator leaves, node order
+  std::vector<int64_t> near_ioff_;           // their item ranges in near_items_h_
+  std::vector<double> sbox_;                 // per node: source bounding box (lo xyz, hi xyz)
+  DeviceBuffer near_order_mx_d_, queue_d_;
   std::vector<int> near_items_h_;
   std::vector<SurfTask> s2m_tasks_h_, s2l_tasks_h_;
   std::vector<int4> s2m_leaf_h_;  // S2M tasks as (leaf, first source, source count)

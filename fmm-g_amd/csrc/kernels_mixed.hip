@@ -63,17 +63,31 @@ __device__ inline void pair_f32_dl(const KParams<float>& p, double dx, double dy
 // NR right-hand sides per launch (see NearArgs): the sources are staged once
 // with all their densities.
 template <int KT, int NR>
-__global__ __launch_bounds__(kNearChunk) void k_near_mx(NearArgs a, KParams<float> pu, KParams<float> pm) {
+struct NearShared {
+  static constexpr int KM = mm_kernel(KT);
+  static constexpr int SD = src_dof(KT), TD = trg_dof(KT), sd = src_dof(KM);
+  static constexpr int MAXD = SD > sd ? SD : sd;
+  static constexpr bool NOR = needs_normals_ct(KT);
+  float sp[kNearChunk * 3];
+  float sn[NOR ? kNearChunk * 3 : 1];
+  double sp64[NOR ? kNearChunk * 3 : 1];
+  float sv[kNearChunk * MAXD * NR];
+  double red[kNearChunk * TD * 2];
+  int next;  // work queue: the chunk this workgroup takes next
+};
+
+template <int KT, int NR>
+__device__ __forceinline__ void near_mx_chunk(const NearArgs a, int chunk, const KParams<float>& pu,
+                                              const KParams<float>& pm, NearShared<KT, NR>& sh) {
   constexpr int KM = mm_kernel(KT);
   constexpr int SD = src_dof(KT), TD = trg_dof(KT), sd = src_dof(KM);
-  constexpr int MAXD = SD > sd ? SD : sd;
   constexpr bool NOR = needs_normals_ct(KT);
-  __shared__ float sp[kNearChunk * 3];
-  __shared__ float sn[NOR ? kNearChunk * 3 : 1];
-  __shared__ double sp64[NOR ? kNearChunk * 3 : 1];
-  __shared__ float sv[kNearChunk * MAXD * NR];
-  __shared__ double red[kNearChunk * TD * 2];
-  const NearChunk ch = a.chunks[blockIdx.x];
+  float* sp = sh.sp;
+  float* sn = sh.sn;
+  double* sp64 = sh.sp64;
+  float* sv = sh.sv;
+  double* red = sh.red;
+  const NearChunk ch = a.chunks[chunk];
   const double4 cb = a.geom[ch.node];
   const int lane = threadIdx.x;
   int S = kNearChunk / max(ch.tnum, 1);
@@ -201,6 +215,25 @@ __global__ __launch_bounds__(kNearChunk) void k_near_mx(NearArgs a, KParams<floa
   }
 }
 
+// Grid or persistent work-queue launch, as k_near_lap_mx.
+template <int KT, int NR>
+__global__ __launch_bounds__(kNearChunk) void k_near_mx(NearArgs a, KParams<float> pu, KParams<float> pm) {
+  __shared__ NearShared<KT, NR> sh;
+  if (!a.queue) {
+    const int c = int(blockIdx.x);
+    near_mx_chunk<KT, NR>(a, a.order ? a.order[c] : c, pu, pm, sh);
+    return;
+  }
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) sh.next = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int c = sh.next;
+    if (c >= a.nchunks) break;
+    near_mx_chunk<KT, NR>(a, a.order ? a.order[c] : c, pu, pm, sh);
+  }
+}
+
 // Laplace single layer (user and equivalent-density kernel 111).  The sources of
 // a chunk's items are packed, item after item, into blocks of kNearChunk float4
 // (x, y, z relative to the leaf centre, density / 4 pi): a lane finds its item
@@ -281,15 +314,28 @@ __device__ __forceinline__ void lap_sweep_nr(const float4* s4, const float* sw, 
 
 constexpr int kItemWin = 128;
 template <int NR>
-__global__ __launch_bounds__(kNearChunk) void k_near_lap_mx(NearArgs a, float pa_u, float pa_m) {
-  __shared__ float4 s4[kNearChunk];
-  __shared__ float sw[NR > 1 ? kNearChunk * NR : 1];
-  __shared__ int pre[kItemWin + 1];
-  __shared__ int ibeg_[kItemWin];
-  __shared__ unsigned char ikind_[kItemWin];
-  __shared__ int wsum[2];
-  __shared__ double red[kNearChunk * 2];
-  const NearChunk ch = a.chunks[blockIdx.x];
+struct LapShared {
+  float4 s4[kNearChunk];
+  float sw[NR > 1 ? kNearChunk * NR : 1];
+  int pre[kItemWin + 1];
+  int ibeg_[kItemWin];
+  unsigned char ikind_[kItemWin];
+  int wsum[2];
+  double red[kNearChunk * 2];
+  int next;  // work queue: the chunk this workgroup takes next
+};
+
+template <int NR>
+__device__ __forceinline__ void near_lap_chunk(const NearArgs a, int chunk, float pa_u, float pa_m,
+                                               LapShared<NR>& sh) {
+  float4* s4 = sh.s4;
+  float* sw = sh.sw;
+  int* pre = sh.pre;
+  int* ibeg_ = sh.ibeg_;
+  unsigned char* ikind_ = sh.ikind_;
+  int* wsum = sh.wsum;
+  double* red = sh.red;
+  const NearChunk ch = a.chunks[chunk];
   const double4 cb = a.geom[ch.node];
   const int lane = threadIdx.x;
   const int tn = max(ch.tnum, 1);
@@ -303,10 +349,15 @@ __global__ __launch_bounds__(kNearChunk) void k_near_lap_mx(NearArgs a, float pa
     tx = float(p[0] - cb.x); ty = float(p[1] - cb.y); tz = float(p[2] - cb.z);
   }
   const int idiv = ch.iend > ch.ibeg && (a.items[ch.iend - 1] & 3) == ITEM_DE ? ch.iend - 1 : ch.iend;
+  // the near (exact, UE) and far (DE) sums; each pass accumulates into pp and
+  // hands it over at its end (static indices: no scratch for a dynamic pass)
   double part[2][NR];
 #pragma unroll
   for (int r = 0; r < NR; r++) part[0][r] = part[1][r] = 0.0;
   for (int pass = 0; pass < 2; pass++) {
+    double pp[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) pp[r] = 0.0;
     const int i0 = pass ? idiv : ch.ibeg, i1 = pass ? ch.iend : idiv;
     for (int w0 = i0; w0 < i1; w0 += kItemWin) {
       const int nw = min(i1 - w0, kItemWin);
@@ -388,7 +439,7 @@ __global__ __launch_bounds__(kNearChunk) void k_near_lap_mx(NearArgs a, float pa
             float acc0 = 0.f, acc1 = 0.f;
             if (cut) lap_sweep(s4, j0, j1, tx, ty, tz, acc0, acc1);
             else lap_sweep_nc(s4, j0, j1, tx, ty, tz, acc0, acc1);
-            part[pass][0] += double(acc0 + acc1);
+            pp[0] += double(acc0 + acc1);
           } else {
             float acc0[NR], acc1[NR];
 #pragma unroll
@@ -396,10 +447,15 @@ __global__ __launch_bounds__(kNearChunk) void k_near_lap_mx(NearArgs a, float pa
             if (cut) lap_sweep_nr<NR, true>(s4, sw, j0, j1, tx, ty, tz, acc0, acc1);
             else lap_sweep_nr<NR, false>(s4, sw, j0, j1, tx, ty, tz, acc0, acc1);
 #pragma unroll
-            for (int r = 0; r < NR; r++) part[pass][r] += double(acc0[r] + acc1[r]);
+            for (int r = 0; r < NR; r++) pp[r] += double(acc0[r] + acc1[r]);
           }
         }
       }
+    }
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      if (pass == 0) part[0][r] = pp[r];
+      else part[1][r] = pp[r];
     }
   }
 #pragma unroll
@@ -419,6 +475,28 @@ __global__ __launch_bounds__(kNearChunk) void k_near_lap_mx(NearArgs a, float pa
       const int64_t o = ch.tbeg + lane;
       a.tval[int64_t(a.r0 + r) * a.tstride + (a.tperm ? int64_t(a.tperm[o]) : o)] = near + far;
     }
+  }
+}
+
+// One workgroup per chunk (a.queue null; chunk blockIdx.x of a.order, or of
+// the chunk list), or a persistent grid whose workgroups take chunks from the
+// global counter *a.queue (zeroed before the launch) in a.order until none is
+// left: every workgroup leaves the loop when the counter passes a.nchunks.
+template <int NR, bool QUEUE>
+__global__ __launch_bounds__(kNearChunk) void k_near_lap_mx(NearArgs a, float pa_u, float pa_m) {
+  __shared__ LapShared<NR> sh;
+  if constexpr (!QUEUE) {
+    const int c = int(blockIdx.x);
+    near_lap_chunk<NR>(a, a.order ? a.order[c] : c, pa_u, pa_m, sh);
+    return;
+  }
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) sh.next = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int c = sh.next;
+    if (c >= a.nchunks) break;
+    near_lap_chunk<NR>(a, a.order ? a.order[c] : c, pa_u, pa_m, sh);
   }
 }
 
@@ -741,25 +819,182 @@ __global__ __launch_bounds__(64) void k_m2l_tile(VlistArgs v, const float2* __re
 }
 
 // ---------------------------------------------------------------------------
+// Vector-kernel Hadamard (sd, td > 1) as the same halo sweep: for colleague
+// offset q and one (source DOF j, target DOF i) component, the 27 translation
+// spectra 2q + [-1,1]^3 sit in registers and serve the 8 x 8 (source child,
+// target child) pairs of all G groups of the tile; the groups' source spectra of
+// DOF j are loaded once per (q, j) and reused for every i.  Per frequency, a
+// translation value feeds 8 G complex multiply-adds (the per-pair kernel
+// k_hadamard_mx reads one per multiply-add).  Pairs of adjacent children fall
+// on the zero spectra of the table's |offset| <= 1 slots, as in k_m2l_tile.
+template <int SD, int TD, int G>
+__global__ __launch_bounds__(64) void k_m2l_tile_vec(VlistArgs v, const float2* __restrict__ dspec,
+                                                     const float2* __restrict__ m2l, float2* __restrict__ espec,
+                                                     float nrm, int ntile, int per_xcd) {
+  const int bx = blockIdx.x;
+  const int tile = (bx & 7) * per_xcd + (bx >> 3);
+  if (tile >= ntile) return;
+  const int H = v.H;
+  const int f = blockIdx.y * 64 + threadIdx.x;
+  const bool fok = f < H;
+  const int fs = fok ? f : H - 1;
+  const int g0 = tile * G;
+  const int ng = min(G, v.ngrp * v.nrhs - g0);
+  __shared__ int shal[G * 216];
+  for (int i = threadIdx.x; i < G * 216; i += 64) {
+    int h = -1;
+    if (i < ng * 216) {
+      const int vg = g0 + i / 216;
+      h = v.halo[int64_t(vg / v.nrhs) * 216 + i % 216];
+      if (h >= 0) h += (vg % v.nrhs) * v.nsrc;
+    }
+    shal[i] = h;
+  }
+  __syncthreads();
+  float2 acc[G][8][TD];
+#pragma unroll
+  for (int gi = 0; gi < G; gi++)
+#pragma unroll
+    for (int t = 0; t < 8; t++)
+#pragma unroll
+      for (int i = 0; i < TD; i++) acc[gi][t][i] = make_float2(0.f, 0.f);
+  for (int q = 0; q < 27; q++) {
+    if (q == 13) continue;  // the parent itself: siblings are adjacent
+    const int qx = q / 9 - 1, qy = (q / 3) % 3 - 1, qz = q % 3 - 1;
+    const int hbase = (2 * qx + 2) * 36 + (2 * qy + 2) * 6 + (2 * qz + 2);
+    const int sbase = (2 * qx + 3) * 49 + (2 * qy + 3) * 7 + (2 * qz + 3);
+    int si[G][8];
+    // translation slots (of the 27) that a present source needs: source child c
+    // meets the 2 x 2 x 2 block of offsets c - tau + 1
+    unsigned need = 0;
+#pragma unroll
+    for (int gi = 0; gi < G; gi++)
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        si[gi][c] = __builtin_amdgcn_readfirstlane(
+            shal[gi * 216 + hbase + (c >> 2) * 36 + ((c >> 1) & 1) * 6 + (c & 1)]);
+        if (si[gi][c] >= 0) {
+          const int b = (c >> 2) * 9 + ((c >> 1) & 1) * 3 + (c & 1);
+          need |= 0x361bu << b;  // offsets {0,1} x {0,1} x {0,1} in base-3 digits from b
+        }
+      }
+    if (!need) continue;  // no source in this colleague: nothing to load
+    for (int j = 0; j < SD; j++) {
+      float2 x[G][8];
+#pragma unroll
+      for (int gi = 0; gi < G; gi++)
+#pragma unroll
+        for (int c = 0; c < 8; c++)
+          x[gi][c] = dspec[(int64_t(si[gi][c] < 0 ? 0 : si[gi][c]) * SD + j) * H + fs];
+#pragma unroll
+      for (int i = 0; i < TD; i++) {
+        float2 T[27];
+#pragma unroll
+        for (int d = 0; d < 27; d++) {
+          const int sl = sbase + (d / 9 - 1) * 49 + ((d / 3) % 3 - 1) * 7 + (d % 3 - 1);
+          T[d] = (need >> d) & 1 ? m2l[(int64_t(sl) * SD * TD + j * TD + i) * H + fs] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int gi = 0; gi < G; gi++)
+#pragma unroll
+          for (int c = 0; c < 8; c++) {
+            if (si[gi][c] < 0) continue;
+            const float2 xv = x[gi][c];
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+              const int d = ((c >> 2) - (t >> 2) + 1) * 9 + (((c >> 1) & 1) - ((t >> 1) & 1) + 1) * 3 +
+                            ((c & 1) - (t & 1) + 1);
+              acc[gi][t][i].x = fmaf(T[d].x, xv.x, fmaf(-T[d].y, xv.y, acc[gi][t][i].x));
+              acc[gi][t][i].y = fmaf(T[d].x, xv.y, fmaf(T[d].y, xv.x, acc[gi][t][i].y));
+            }
+          }
+      }
+    }
+  }
+  if (!fok) return;
+#pragma unroll
+  for (int gi = 0; gi < G; gi++) {
+    if (gi >= ng) break;
+    const int vg = g0 + gi;
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+      int tg = v.tgt[(vg / v.nrhs) * 8 + t];
+      if (tg < 0) continue;
+      tg += (vg % v.nrhs) * v.ntrg;
+#pragma unroll
+      for (int i = 0; i < TD; i++)
+        espec[(int64_t(tg) * TD + i) * H + f] = make_float2(acc[gi][t][i].x * nrm, acc[gi][t][i].y * nrm);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 #define KIFMM_USER_KERNELS_MX(X) X(111) X(121) X(311) X(312) X(321) X(322) X(511) X(521)
+
+__global__ void k_zero_int(int* p) { *p = 0; }
+
+// Resident workgroups of a near kernel over the whole device (persistent grid).
+template <typename K>
+static unsigned persistent_blocks(K kernel) {
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kNearChunk, 0) != hipSuccess)
+    return 0;
+  return unsigned(std::max(1, cus) * std::max(1, per));
+}
 
 hipError_t launch_near_mixed(int kt, const NearArgs& a0, const KParams<double>& pu, const KParams<double>& pm,
                              hipStream_t s) {
   if (a0.nchunks <= 0) return hipSuccess;
   if (a0.nrhs < 1) return hipErrorInvalidValue;
   const KParams<float> fu = to_f32(pu), fm = to_f32(pm);
-  const dim3 g(unsigned(a0.nchunks)), b(kNearChunk);
+  const dim3 b(kNearChunk);
   for (int r0 = 0; r0 < a0.nrhs; r0 += kMaxNr) {
     NearArgs a = a0;
     a.r0 = r0;
     a.nr = std::min(kMaxNr, a0.nrhs - r0);
     const int nr = rhs_group(a.nr);
+    dim3 g(unsigned(a0.nchunks));
+    if (a.queue) {
+      // a grid of resident workgroups over the work queue (at most one per chunk)
+      unsigned pb = 0;
+      if (kt == 111) {
+        switch (nr) {
+          case 1: pb = persistent_blocks(k_near_lap_mx<1, true>); break;
+          case 2: pb = persistent_blocks(k_near_lap_mx<2, true>); break;
+          case 4: pb = persistent_blocks(k_near_lap_mx<4, true>); break;
+          default: pb = persistent_blocks(k_near_lap_mx<8, true>); break;
+        }
+      } else {
+        switch (kt * 16 + nr) {
+#define KIFMM_CASE_NR(K, R) case K * 16 + R: pb = persistent_blocks(k_near_mx<K, R>); break;
+#define KIFMM_CASE(K) KIFMM_CASE_NR(K, 1) KIFMM_CASE_NR(K, 2) KIFMM_CASE_NR(K, 4) KIFMM_CASE_NR(K, 8)
+          KIFMM_USER_KERNELS_MX(KIFMM_CASE)
+#undef KIFMM_CASE
+#undef KIFMM_CASE_NR
+          default: return hipErrorInvalidValue;
+        }
+      }
+      if (pb == 0) return hipErrorInvalidValue;
+      g = dim3(std::min(pb, unsigned(a0.nchunks)));
+      k_zero_int<<<1, 1, 0, s>>>(a.queue);
+    }
     if (kt == 111) {
-      switch (nr) {
-        case 1: k_near_lap_mx<1><<<g, b, 0, s>>>(a, fu.a, fm.a); break;
-        case 2: k_near_lap_mx<2><<<g, b, 0, s>>>(a, fu.a, fm.a); break;
-        case 4: k_near_lap_mx<4><<<g, b, 0, s>>>(a, fu.a, fm.a); break;
-        default: k_near_lap_mx<8><<<g, b, 0, s>>>(a, fu.a, fm.a); break;
+      if (a.queue) {
+        switch (nr) {
+          case 1: k_near_lap_mx<1, true><<<g, b, 0, s>>>(a, fu.a, fm.a); break;
+          case 2: k_near_lap_mx<2, true><<<g, b, 0, s>>>(a, fu.a, fm.a); break;
+          case 4: k_near_lap_mx<4, true><<<g, b, 0, s>>>(a, fu.a, fm.a); break;
+          default: k_near_lap_mx<8, true><<<g, b, 0, s>>>(a, fu.a, fm.a); break;
+        }
+      } else {
+        switch (nr) {
+          case 1: k_near_lap_mx<1, false><<<g, b, 0, s>>>(a, fu.a, fm.a); break;
+          case 2: k_near_lap_mx<2, false><<<g, b, 0, s>>>(a, fu.a, fm.a); break;
+          case 4: k_near_lap_mx<4, false><<<g, b, 0, s>>>(a, fu.a, fm.a); break;
+          default: k_near_lap_mx<8, false><<<g, b, 0, s>>>(a, fu.a, fm.a); break;
+        }
       }
     } else {
       switch (kt * 16 + nr) {
@@ -816,10 +1051,28 @@ hipError_t launch_hadamard_mixed(const VlistArgs& v, const float2* dspec, const 
     else
       k_m2l_tile<false><<<grid2, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm, ntile, per_xcd);
   }
-  else if (v.sd == 4 && v.td == 3) k_hadamard_mx<4, 3><<<grid, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm);
-  else if (v.sd == 3 && v.td == 3) k_hadamard_mx<3, 3><<<grid, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm);
-  else if (v.sd == 3 && v.td == 4) k_hadamard_mx<3, 4><<<grid, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm);
-  else return hipErrorInvalidValue;
+  else if (v.had_form == 0) {  // one wave per group, translation spectra read per pair
+    if (v.sd == 4 && v.td == 3) k_hadamard_mx<4, 3><<<grid, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm);
+    else if (v.sd == 3 && v.td == 3) k_hadamard_mx<3, 3><<<grid, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm);
+    else if (v.sd == 3 && v.td == 4) k_hadamard_mx<3, 4><<<grid, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm);
+    else return hipErrorInvalidValue;
+  } else {  // halo sweep over tiles of 1 or 2 groups
+    const int G = v.had_form == 1 ? 1 : 2;
+    const int ntile = int((nvg + G - 1) / G);
+    const int per_xcd = (ntile + 7) / 8;
+    const dim3 grid2(unsigned(8 * per_xcd), unsigned((v.H + 63) / 64));
+#define KIFMM_TILE(SD_, TD_)                                                                                \
+  if (v.sd == SD_ && v.td == TD_) {                                                                         \
+    if (G == 1) k_m2l_tile_vec<SD_, TD_, 1><<<grid2, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm, ntile, per_xcd); \
+    else k_m2l_tile_vec<SD_, TD_, 2><<<grid2, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm, ntile, per_xcd);        \
+    return hipGetLastError();                                                                               \
+  }
+    KIFMM_TILE(4, 3)
+    KIFMM_TILE(3, 3)
+    KIFMM_TILE(3, 4)
+#undef KIFMM_TILE
+    return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -24,6 +24,7 @@ H = C.c_void_p
 
 PREC = {"fp64": 0, "mixed": 1}
 PHASES = ["EvlIni", "EvlUpwComp", "EvlVList", "EvlXList", "EvlCombine", "EvlUList", "EvlFinalize", "VListHadamard"]
+TUNING = ["near_mode", "near_stage", "translations", "vec_hadamard"]
 SETUP_PARTS = ["tree", "tree_device", "tree_copy", "operators", "tasks", "upload"]
 WORK = ["near_pairs", "near_exact_pairs", "s2m_pairs", "s2l_pairs", "vlist_pairs", "n_vsrc",
         "n_vtrg", "n_vgrp", "up_cols", "m2m_cols", "dn_cols", "l2l_cols"]
@@ -65,6 +66,11 @@ SIGNATURES = {
     "kifmm_work_counts": (C.c_int, [H, P_D, C.c_int]),
     "kifmm_set_tree_builder": (C.c_int, [H, C.c_int]),
     "kifmm_setup_times": (C.c_int, [H, P_D, C.c_int]),
+    "kifmm_set_translation": (C.c_int, [H, C.c_int]),
+    "kifmm_autotune": (C.c_int, [H, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+    "kifmm_get_tuning": (C.c_int, [H, P_I, C.c_int]),
+    "kifmm_set_tuning": (C.c_int, [H, P_I, C.c_int]),
+    "kifmm_tuning_report": (C.c_int, [H, P_D, C.c_int]),
     "kifmm_options_lookup": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int]),
     "kifmm_last_error": (C.c_char_p, [H]),
 }
@@ -383,3 +389,35 @@ class FMM3d:
         ms = np.zeros(len(SETUP_PARTS))
         self._chk(lib().kifmm_setup_times(self._h, _pd(ms), len(SETUP_PARTS)))
         return dict(zip(SETUP_PARTS, ms.tolist()))
+
+    def set_translation(self, terms):
+        """Mixed-path translations: 0 FP64 matrix cores (M2M fused into the
+        pseudo-inverse), 3 or 6 split-BF16 products; setup() again afterwards."""
+        self._chk(lib().kifmm_set_translation(self._h, int(terms)))
+
+    def autotune(self, d_src_den, d_trg_val, nrhs=1, stream=None):
+        """Times the mixed path's launch-shape choices on this tree and keeps the
+        fastest (device tensors, nrhs right-hand sides); returns the chosen
+        knobs and every trial as (knob, value, ms)."""
+        self._chk(lib().kifmm_autotune(self._h, _dev(d_src_den, nrhs * self.n_src * self.src_dof, "densities"),
+                                       _dev(d_trg_val, nrhs * self.n_trg * self.trg_dof, "values"), int(nrhs),
+                                       _stream(stream)))
+        n = lib().kifmm_tuning_report(self._h, None, 0)
+        rows = np.zeros(3 * max(n, 1))
+        lib().kifmm_tuning_report(self._h, _pd(rows), n)
+        trials = [(TUNING[int(rows[3 * i])], int(rows[3 * i + 1]), float(rows[3 * i + 2])) for i in range(n)]
+        return {"tuning": self.tuning(), "trials": trials}
+
+    def tuning(self):
+        k = (C.c_int * len(TUNING))()
+        self._chk(lib().kifmm_get_tuning(self._h, k, len(TUNING)))
+        return dict(zip(TUNING, list(k)))
+
+    def set_tuning(self, near_mode=None, near_stage=None, translations=None, vec_hadamard=None):
+        cur = self.tuning()
+        for name, v in (("near_mode", near_mode), ("near_stage", near_stage), ("translations", translations),
+                        ("vec_hadamard", vec_hadamard)):
+            if v is not None:
+                cur[name] = int(v)
+        k = (C.c_int * len(TUNING))(*[cur[n] for n in TUNING])
+        self._chk(lib().kifmm_set_tuning(self._h, k, len(TUNING)))

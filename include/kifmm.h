@@ -181,6 +181,42 @@ int kifmm_set_tree_builder(kifmm_handle h, int on_device);
 #define KIFMM_NSETUP 6
 int kifmm_setup_times(kifmm_handle h, double *ms, int n);
 
+/* Mixed-precision translations (M2M, matmgnt3d_mpi.cpp:151-193, and L2L,
+ * :231-274; the reference applies them as FP64 GEMVs): terms = 0 (default)
+ * runs them on the FP64 matrix cores with M2M fused into the upward
+ * pseudo-inverse; 3 or 6 runs them as split-BF16 products (BF16x3, BF16x6) on
+ * the BF16 matrix cores, accumulated in FP32, with the equivalent densities
+ * scaled by their level's 2^(l deg) before the split.  The pseudo-inverse
+ * applications stay FP64 either way.  No effect on the FP64 path.  Takes
+ * effect at the next kifmm_setup (the handle needs setup again). */
+int kifmm_set_translation(kifmm_handle h, int terms);
+
+/* Autotuner of the mixed path's launch shapes (no reference counterpart; the
+ * reference has no launch shapes to choose).  Times each choice on this
+ * handle's own tree with the caller's device densities (nrhs right-hand sides,
+ * as kifmm_evaluate_multi_device; d_trg_val receives values) and keeps the
+ * fastest, one choice at a time:
+ *   knob 2: translations, 0 (FP64 matrix cores, fused M2M) or 6 (split BF16x6),
+ *           timed over the whole evaluate;
+ *   knob 1: staging weight of the near-field chunk split, index 0..2 of
+ *           {0.04, 0.15, 0.30}, timed on the near-field phase;
+ *   knob 0: near-field launch, 0 one workgroup per chunk in leaf order, 1 the
+ *           same largest chunk first, 2 a persistent grid over a global work
+ *           queue, largest chunk first; timed on the near-field phase;
+ *   knob 3: V-list Hadamard of the vector kernels (Stokes, Navier; not tuned
+ *           for Laplace), 0 translation spectra read per pair, 1 and 2 the halo
+ *           sweep over tiles of 1 or 2 sibling groups; timed on the V list.
+ * Synchronous.  No-op (success) for the FP64 path, whose summation order is
+ * fixed.  kifmm_get_tuning / kifmm_set_tuning read and apply the knobs
+ * (KIFMM_NTUNE ints, index = knob); kifmm_tuning_report fills rows of
+ * (knob, value, ms) for every trial of the last autotune and returns their
+ * number (rows may be NULL to count only). */
+#define KIFMM_NTUNE 4
+int kifmm_autotune(kifmm_handle h, const double *d_src_den, double *d_trg_val, int nrhs, void *stream);
+int kifmm_get_tuning(kifmm_handle h, int *knobs, int n);
+int kifmm_set_tuning(kifmm_handle h, const int *knobs, int n);
+int kifmm_tuning_report(kifmm_handle h, double *rows, int maxrows);
+
 /* Operator tables of MatMgnt3d_MPI (matmgnt3d_mpi.cpp:107-431), column-major
  * FP64: which 0 = UC->UE pinv, 1 = DC->DE pinv, 2 = 8 M2M, 3 = 8 L2L,
  * 4 = M2L spectra [343][sd*td][H] complex.  With out == NULL only *count
