@@ -216,7 +216,14 @@ class Runner:
         f.set_points(pos, nor)
         f.setup()
         setup_s = time.perf_counter() - t0
+        setup_parts = f.setup_times()
         d_val = torch.empty(n * f.trg_dof, dtype=torch.float64, device=self.dev)
+        tuning = None
+        if precision == "mixed":
+            # launch shapes chosen on this tree (untimed, like the setup)
+            t0 = time.perf_counter()
+            tuning = f.autotune(d_den, d_val)
+            tuning["seconds"] = round(time.perf_counter() - t0, 3)
         torch.cuda.synchronize()
         for _ in range(warmup):
             f.evaluate_device(d_den, d_val)
@@ -266,7 +273,7 @@ class Runner:
               % (tag, precision, np_, setup_s, dt * 1e3, dt_graph * 1e3, rel_err, direct_ms,
                  json.dumps({k: round(v, 3) for k, v in phases.items()})), file=sys.stderr, flush=True)
         return {"f": f, "dt": dt, "dt_graph": dt_graph, "phases": phases, "rel_err": rel_err, "val": val,
-                "setup_s": setup_s, "direct_ms": direct_ms}
+                "setup_s": setup_s, "setup_parts": setup_parts, "direct_ms": direct_ms, "tuning": tuning}
 
 
 def pair_summary(mx, f64, n):
@@ -298,6 +305,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=262144)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--numchk", type=int, default=4096)
+    ap.add_argument("--c5-problems", type=int, default=512,
+                    help="problems of the batched Navier workload (the first k of the 512)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="copy 512 MiB device-to-device first (calibrates FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -400,6 +409,8 @@ def main():
         "speedup_vs_fp64": round(dt64 / dt, 3) if dt64 else None,
         "rel_diff_vs_fp64": diff_fp64,
         "setup_s": round(r["setup_s"], 3),
+        "setup_parts_ms": {k: round(v, 2) for k, v in r["setup_parts"].items()},
+        "tuning": r["tuning"],
         "direct_sum_ms": round(r["direct_ms"], 2),
         "phases_ms": {k: round(v, 4) for k, v in phases.items()},
         "roofline": roofline,
@@ -444,6 +455,7 @@ def main():
         wb = b["f"].work_counts()
         pr, tr = phase_roofline(321, 6, wb, b["phases"], True, c[6], c[6])
         s["phase_roofline_mixed"] = pr
+        s["tuning"] = b["tuning"]
         s["eval_roofline_frac_mixed"] = round(
             tr / (sum(b["phases"][k] for k in b["phases"] if k != "VListHadamard") * 1e-3), 4)
         extra["c2"] = s
@@ -451,6 +463,8 @@ def main():
         del p2, n2_, dn2, dd2
         # the gradient workload: forward, transposed and field passes
         extra["c4"] = bench_gradient(R, xs)
+        # the batched Navier workload: 512 problems x 8 right-hand sides
+        extra["c5"] = bench_batched(R, args.c5_problems)
         out.update(extra)
     print(json.dumps(out), flush=True)
 
@@ -485,6 +499,108 @@ def bench_gradient(R, steps):
         res[prec + "_ms"] = times
         print("[c4 %s] %s" % (prec, json.dumps(times)), file=sys.stderr, flush=True)
         f.close()
+    return res
+
+
+C5_POINTS_PER_FOREST = 3_500_000
+
+
+def c5_problems(nprob=512):
+    """Point counts and points of the batched workload: N_p = round(exp(U[ln 2048,
+    ln 262144])), uniform in [0,1)^3, one seeded stream per problem."""
+    from kifmm import workloads as W
+    rng = np.random.default_rng(W.seed(4, 0))
+    sizes = np.rint(np.exp(rng.uniform(np.log(2048), np.log(262144), 512))).astype(np.int64)[:nprob]
+    return sizes, [W.uniform(int(m), W.seed(4, 0) * 1000 + p) for p, m in enumerate(sizes)]
+
+
+def bench_batched(R, nprob):
+    """Navier single layer (mu 1, nu 0.3), np 8, ptsMax 120, maxLevel 10, 8
+    right-hand sides of 3-vector N(0,1) densities per problem.  The problems go
+    into forests of at most 3.5M points (the V-list spectra of 8 right-hand
+    sides of all 512 problems, about 0.7 TB, do not fit in HBM at once); each
+    forest is one handle whose every pass is one launch over all its problems
+    and right-hand sides.  The time of one evaluation of the whole workload is
+    the sum over the forests of a warm 8-RHS evaluate (mixed path, after the
+    handle's autotune); the FP64 path is timed on the first forest.  Error:
+    FP64 direct sum at 256 targets of the first problem, first right-hand
+    side."""
+    import torch
+    import kifmm
+    from kifmm import workloads as W
+    sizes, pts = c5_problems(nprob)
+    forests, cur, acc = [], [], 0
+    for p, m in enumerate(sizes):
+        if cur and acc + m > C5_POINTS_PER_FOREST:
+            forests.append(cur)
+            cur, acc = [], 0
+        cur.append(p)
+        acc += int(m)
+    if cur:
+        forests.append(cur)
+    NRHS = 8
+    res = {"workload": "Navier-S (511, nu 0.3) np=8 ptsMax=120 maxLevel=10, %d problems (N_p log-uniform in "
+                       "[2048, 262144], total %d points) x %d RHS" % (len(sizes), int(sizes.sum()), NRHS),
+           "n_points": int(sizes.sum()), "n_forests": len(forests), "nrhs": NRHS}
+    t_mixed, t_setup, tune = 0.0, 0.0, None
+    for fi, probs in enumerate(forests):
+        for prec in (("mixed", "fp64") if fi == 0 else ("mixed",)):
+            f = kifmm.FMM3d(511, [1.0, 0.3], 8, 120, 10, precision=prec)
+            f.set_points_batch([pts[p] for p in probs])
+            t0 = time.perf_counter()
+            f.setup()
+            ts = time.perf_counter() - t0
+            n = f.n_src
+            den = np.concatenate([W.densities(n, 3, W.seed(4, 1) * 100 + 10 * fi + r, kind="normal")
+                                  for r in range(NRHS)])
+            d_den = torch.from_numpy(den).to(R.dev)
+            d_val = torch.empty(NRHS * n * 3, dtype=torch.float64, device=R.dev)
+            if prec == "mixed":
+                tune = f.autotune(d_den, d_val, nrhs=NRHS)["tuning"] if fi == 0 else tune
+                if fi > 0:
+                    f.set_tuning(**tune)
+            f.evaluate_multi_device(d_den, d_val, NRHS)  # warm
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            f.evaluate_multi_device(d_den, d_val, NRHS)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            if fi == 0:
+                m0 = int(sizes[probs[0]])
+                idx = np.random.default_rng(W.seed(4, 3)).choice(m0, 256, replace=False)
+                g = kifmm.FMM3d(511, [1.0, 0.3], 8, 120, 10)
+                g.set_points(pts[probs[0]])
+                g.setup()
+                d_chk = torch.from_numpy(np.ascontiguousarray(pts[probs[0]][idx])).to(R.dev)
+                d_ref = torch.empty(256 * 3, dtype=torch.float64, device=R.dev)
+                g.direct_device(d_den[:m0 * 3].contiguous(), d_chk, d_ref)
+                ref = d_ref.cpu().numpy().reshape(256, 3)
+                got = d_val[:m0 * 3].cpu().numpy().reshape(m0, 3)[idx]
+                res["rel_err_%s" % prec] = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+                g.close()
+            if prec == "mixed":
+                t_mixed += dt
+                t_setup += ts
+            else:
+                res["fp64_first_forest_ms"] = round(dt * 1e3, 3)
+                res["fp64_first_forest_setup_s"] = round(ts, 3)
+            if fi == 0:
+                res["%s_first_forest_ms" % prec] = round(dt * 1e3, 3)
+                res["first_forest"] = {"problems": len(probs), "points": int(n)}
+                f.set_timing(True)
+                f.evaluate_multi_device(d_den, d_val, NRHS)
+                res["%s_first_forest_phases_ms" % prec] = {k: round(v, 3) for k, v in f.phase_times().items()}
+                f.set_timing(False)
+            print("[c5 forest %d/%d %s] %d problems %d points setup %.2fs evaluate(8 RHS) %.1f ms"
+                  % (fi + 1, len(forests), prec, len(probs), n, ts, dt * 1e3), file=sys.stderr, flush=True)
+            f.close()
+            del d_den, d_val
+            gc.collect()
+            torch.cuda.empty_cache()
+    res["mixed_ms"] = round(t_mixed * 1e3, 3)
+    res["mixed_mpoints_s"] = round(res["n_points"] * NRHS / t_mixed / 1e6, 3)  # point-RHS per second
+    res["setup_s"] = round(t_setup, 3)
+    res["tuning"] = tune
     return res
 
 
