@@ -186,8 +186,9 @@ def phase_roofline(kt, np_, w, phases, mixed, n_src, n_trg):
 
 
 class Runner:
-    def __init__(self, dev, steps, warmup, numchk):
+    def __init__(self, dev, steps, warmup, numchk, autotune=True):
         self.dev, self.steps, self.warmup, self.numchk = dev, steps, warmup, numchk
+        self.autotune = autotune
 
     def inputs(self, cfg, n):
         from kifmm import workloads as W
@@ -219,7 +220,7 @@ class Runner:
         setup_parts = f.setup_times()
         d_val = torch.empty(n * f.trg_dof, dtype=torch.float64, device=self.dev)
         tuning = None
-        if precision == "mixed":
+        if precision == "mixed" and self.autotune:
             # launch shapes chosen on this tree (untimed, like the setup)
             t0 = time.perf_counter()
             tuning = f.autotune(d_den, d_val)
@@ -305,6 +306,10 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=262144)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--numchk", type=int, default=4096)
+    ap.add_argument("--only", choices=["c2", "c4", "c5"], default=None,
+                    help="run one secondary workload alone and print its JSON (profiling runs)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="time the mixed path with its default launch shapes (profiling runs)")
     ap.add_argument("--c5-problems", type=int, default=512,
                     help="problems of the batched Navier workload (the first k of the 512)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -324,7 +329,17 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    R = Runner(dev, args.steps, args.warmup, args.numchk)
+    R = Runner(dev, args.steps, args.warmup, args.numchk, autotune=not args.no_autotune)
+    if args.only:
+        # one secondary workload alone (profiling runs); prints its JSON object
+        if args.only == "c2":
+            res = bench_spheres(R, min(args.steps, 5))
+        elif args.only == "c4":
+            res = bench_gradient(R, min(args.steps, 5))
+        else:
+            res = bench_batched(R, args.c5_problems)
+        print(json.dumps({args.only: res}), flush=True)
+        return
 
     # ---- headline: the 8M-point clustered Laplace workload, np 6 ----
     cfg = CONFIGS["c3"]
@@ -443,30 +458,34 @@ def main():
                        "mpoints_s": round(c[6] / a["dt"] / 1e6, 3), "rel_err": a["rel_err"]}
         release(a)
         # Stokes double layer on 32 spheres, FP64 and mixed
-        c = CONFIGS["c2"]
-        p2, n2_, dn2, dd2, i2 = R.inputs(c, c[6])
-        b64 = R.run(c, c[6], "fp64", p2, n2_, dd2, i2, steps=xs, warmup=1, tag="c2")
-        b64["f"].close()
-        b = R.run(c, c[6], "mixed", p2, n2_, dd2, i2, steps=xs, warmup=1, tag="c2")
-        s = pair_summary(b, b64, c[6])
-        s["workload"] = WORKLOAD_NAME["c2"]
-        s["phases_ms_mixed"] = {k: round(v, 4) for k, v in b["phases"].items()}
-        s["phases_ms_fp64"] = {k: round(v, 4) for k, v in b64["phases"].items()}
-        wb = b["f"].work_counts()
-        pr, tr = phase_roofline(321, 6, wb, b["phases"], True, c[6], c[6])
-        s["phase_roofline_mixed"] = pr
-        s["tuning"] = b["tuning"]
-        s["eval_roofline_frac_mixed"] = round(
-            tr / (sum(b["phases"][k] for k in b["phases"] if k != "VListHadamard") * 1e-3), 4)
-        extra["c2"] = s
-        release(b, b64)
-        del p2, n2_, dn2, dd2
+        extra["c2"] = bench_spheres(R, xs)
         # the gradient workload: forward, transposed and field passes
         extra["c4"] = bench_gradient(R, xs)
         # the batched Navier workload: 512 problems x 8 right-hand sides
         extra["c5"] = bench_batched(R, args.c5_problems)
         out.update(extra)
     print(json.dumps(out), flush=True)
+
+
+def bench_spheres(R, xs):
+    """Stokes double layer on 32 spheres, 1,048,576 points, np 6: FP64 and mixed."""
+    c = CONFIGS["c2"]
+    p2, n2_, dn2, dd2, i2 = R.inputs(c, c[6])
+    b64 = R.run(c, c[6], "fp64", p2, n2_, dd2, i2, steps=xs, warmup=1, tag="c2")
+    b64["f"].close()
+    b = R.run(c, c[6], "mixed", p2, n2_, dd2, i2, steps=xs, warmup=1, tag="c2")
+    s = pair_summary(b, b64, c[6])
+    s["workload"] = WORKLOAD_NAME["c2"]
+    s["phases_ms_mixed"] = {k: round(v, 4) for k, v in b["phases"].items()}
+    s["phases_ms_fp64"] = {k: round(v, 4) for k, v in b64["phases"].items()}
+    wb = b["f"].work_counts()
+    pr, tr = phase_roofline(321, 6, wb, b["phases"], True, c[6], c[6])
+    s["phase_roofline_mixed"] = pr
+    s["tuning"] = b["tuning"]
+    s["eval_roofline_frac_mixed"] = round(
+        tr / (sum(b["phases"][k] for k in b["phases"] if k != "VListHadamard") * 1e-3), 4)
+    release(b, b64)
+    return s
 
 
 def bench_gradient(R, steps):
@@ -555,7 +574,7 @@ def bench_batched(R, nprob):
                                   for r in range(NRHS)])
             d_den = torch.from_numpy(den).to(R.dev)
             d_val = torch.empty(NRHS * n * 3, dtype=torch.float64, device=R.dev)
-            if prec == "mixed":
+            if prec == "mixed" and R.autotune:
                 tune = f.autotune(d_den, d_val, nrhs=NRHS)["tuning"] if fi == 0 else tune
                 if fi > 0:
                     f.set_tuning(**tune)
@@ -589,8 +608,18 @@ def bench_batched(R, nprob):
                 res["first_forest"] = {"problems": len(probs), "points": int(n)}
                 f.set_timing(True)
                 f.evaluate_multi_device(d_den, d_val, NRHS)
-                res["%s_first_forest_phases_ms" % prec] = {k: round(v, 3) for k, v in f.phase_times().items()}
+                ph = f.phase_times()
+                res["%s_first_forest_phases_ms" % prec] = {k: round(v, 3) for k, v in ph.items()}
                 f.set_timing(False)
+                # Hadamard roofline: 8 flops per complex multiply-add, 9 components
+                # (sd = td = 3), H = 2304 frequencies at np 8, per V pair and RHS
+                w = f.work_counts()
+                flops = 8.0 * 2304 * 9 * w["vlist_pairs"] * NRHS
+                peak = PEAK_TF["fp32" if prec == "mixed" else "fp64"]
+                tf = flops / (ph["VListHadamard"] * 1e-3) / 1e12
+                res["%s_first_forest_hadamard" % prec] = {
+                    "vlist_pairs": w["vlist_pairs"], "algorithmic_flops": flops, "achieved_tflops": round(tf, 3),
+                    "peak_tflops": peak, "frac": round(tf / peak, 4)}
             print("[c5 forest %d/%d %s] %d problems %d points setup %.2fs evaluate(8 RHS) %.1f ms"
                   % (fi + 1, len(forests), prec, len(probs), n, ts, dt * 1e3), file=sys.stderr, flush=True)
             f.close()

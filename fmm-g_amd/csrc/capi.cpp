@@ -13,6 +13,9 @@
 
 struct kifmm_fmm {
   kifmm::FMM3d fmm;
+  // staging of the host-pointer entry points: kept and grown, so repeated calls
+  // allocate nothing
+  kifmm::DeviceBuffer in, out, aux;
 };
 
 namespace {
@@ -22,6 +25,18 @@ int set_err(kifmm_handle h, int code, const std::string& msg) {
   if (h) h->fmm.setError(msg);
   else g_create_error = msg;
   return code;
+}
+
+// Device staging for host arrays: grows b to at least `bytes` (contents not kept).
+hipError_t grow(kifmm::DeviceBuffer& b, size_t bytes) {
+  return b.bytes() >= bytes ? hipSuccess : b.alloc(bytes);
+}
+
+// Copies `bytes` of host data into the (grown) staging buffer b.
+hipError_t stage_in(kifmm::DeviceBuffer& b, const void* host, size_t bytes) {
+  hipError_t e = grow(b, bytes);
+  if (e != hipSuccess || bytes == 0) return e;
+  return hipMemcpy(b.as<void>(), host, bytes, hipMemcpyHostToDevice);
 }
 
 // SplitMix64: sample indices for kifmm_check.
@@ -140,10 +155,10 @@ int kifmm_evaluate(kifmm_handle h, const double* src_den, double* trg_val) {
   if (!f.isSetup()) return set_err(h, KIFMM_ERR_STATE, "evaluate called before setup");
   if ((f.numSrc() > 0 && !src_den) || (f.numTrg() > 0 && !trg_val)) return set_err(h, KIFMM_ERR_ARG, "null array");
   const size_t nd = size_t(f.numSrc()) * f.srcDOF(), nv = size_t(f.numTrg()) * f.trgDOF();
-  kifmm::DeviceBuffer d, v;
+  kifmm::DeviceBuffer &d = h->in, &v = h->out;
   hipError_t e;
-  if ((e = d.upload(src_den, sizeof(double) * nd)) != hipSuccess) return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
-  if ((e = v.alloc(sizeof(double) * nv)) != hipSuccess) return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
+  if ((e = stage_in(d, src_den, sizeof(double) * nd)) != hipSuccess) return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
+  if ((e = grow(v, sizeof(double) * nv)) != hipSuccess) return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
   int rc = f.evaluate(d.as<double>(), v.as<double>(), nullptr);
   if (rc) return rc;
   if ((e = hipMemcpy(trg_val, v.as<double>(), sizeof(double) * nv, hipMemcpyDeviceToHost)) != hipSuccess)
@@ -166,10 +181,10 @@ int kifmm_evaluate_multi(kifmm_handle h, const double* src_den, double* trg_val,
   if (nrhs == 0) return KIFMM_OK;
   if ((f.numSrc() > 0 && !src_den) || (f.numTrg() > 0 && !trg_val)) return set_err(h, KIFMM_ERR_ARG, "null array");
   const size_t nd = size_t(f.numSrc()) * f.srcDOF() * nrhs, nv = size_t(f.numTrg()) * f.trgDOF() * nrhs;
-  kifmm::DeviceBuffer d, v;
+  kifmm::DeviceBuffer &d = h->in, &v = h->out;
   hipError_t e;
-  if ((e = d.upload(src_den, sizeof(double) * nd)) != hipSuccess) return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
-  if ((e = v.alloc(sizeof(double) * nv)) != hipSuccess) return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
+  if ((e = stage_in(d, src_den, sizeof(double) * nd)) != hipSuccess) return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
+  if ((e = grow(v, sizeof(double) * nv)) != hipSuccess) return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
   int rc = kifmm_evaluate_multi_device(h, d.as<double>(), v.as<double>(), nrhs, nullptr);
   if (rc) return rc;
   if ((e = hipMemcpy(trg_val, v.as<double>(), sizeof(double) * nv, hipMemcpyDeviceToHost)) != hipSuccess)
@@ -188,10 +203,10 @@ int kifmm_evaluate_adjoint(kifmm_handle h, const double* trg_in, double* src_out
   if (!f.isSetup()) return set_err(h, KIFMM_ERR_STATE, "evaluate_adjoint called before setup");
   if ((f.numTrg() > 0 && !trg_in) || (f.numSrc() > 0 && !src_out)) return set_err(h, KIFMM_ERR_ARG, "null array");
   const size_t nw = size_t(f.numTrg()) * f.trgDOF(), ns = size_t(f.numSrc()) * f.srcDOF();
-  kifmm::DeviceBuffer w, o;
+  kifmm::DeviceBuffer &w = h->in, &o = h->out;
   hipError_t e;
-  if ((e = w.upload(trg_in, sizeof(double) * nw)) != hipSuccess) return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
-  if ((e = o.alloc(sizeof(double) * ns)) != hipSuccess) return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
+  if ((e = stage_in(w, trg_in, sizeof(double) * nw)) != hipSuccess) return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
+  if ((e = grow(o, sizeof(double) * ns)) != hipSuccess) return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
   int rc = f.evaluateAdjoint(w.as<double>(), o.as<double>(), nullptr);
   if (rc) return rc;
   if ((e = hipMemcpy(src_out, o.as<double>(), sizeof(double) * ns, hipMemcpyDeviceToHost)) != hipSuccess)
@@ -210,10 +225,10 @@ int kifmm_field(kifmm_handle h, const double* src_den, double* trg_grad) {
   if (!f.isSetup()) return set_err(h, KIFMM_ERR_STATE, "field called before setup");
   if ((f.numSrc() > 0 && !src_den) || (f.numTrg() > 0 && !trg_grad)) return set_err(h, KIFMM_ERR_ARG, "null array");
   const size_t nd = size_t(f.numSrc()) * f.srcDOF(), ng = size_t(f.numTrg()) * 3;
-  kifmm::DeviceBuffer d, g;
+  kifmm::DeviceBuffer &d = h->in, &g = h->out;
   hipError_t e;
-  if ((e = d.upload(src_den, sizeof(double) * nd)) != hipSuccess) return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
-  if ((e = g.alloc(sizeof(double) * ng)) != hipSuccess) return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
+  if ((e = stage_in(d, src_den, sizeof(double) * nd)) != hipSuccess) return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
+  if ((e = grow(g, sizeof(double) * ng)) != hipSuccess) return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
   int rc = f.field(d.as<double>(), g.as<double>(), nullptr);
   if (rc) return rc;
   if ((e = hipMemcpy(trg_grad, g.as<double>(), sizeof(double) * ng, hipMemcpyDeviceToHost)) != hipSuccess)
@@ -271,13 +286,13 @@ int kifmm_check(kifmm_handle h, const double* src_den, const double* trg_val, in
   std::vector<double> chk_pos(3 * idx.size()), chk_val(TD * idx.size());
   for (size_t k = 0; k < idx.size(); k++)
     for (int d = 0; d < 3; d++) chk_pos[3 * k + d] = tp[3 * idx[k] + d];
-  kifmm::DeviceBuffer dd, dp, dv;
+  kifmm::DeviceBuffer &dd = h->in, &dp = h->aux, &dv = h->out;
   hipError_t e;
   const size_t nd = size_t(f.numSrc()) * f.srcDOF();
-  if ((e = dd.upload(src_den, sizeof(double) * nd)) != hipSuccess) return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
-  if ((e = dp.upload(chk_pos.data(), sizeof(double) * chk_pos.size())) != hipSuccess)
+  if ((e = stage_in(dd, src_den, sizeof(double) * nd)) != hipSuccess) return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
+  if ((e = stage_in(dp, chk_pos.data(), sizeof(double) * chk_pos.size())) != hipSuccess)
     return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
-  if ((e = dv.alloc(sizeof(double) * chk_val.size())) != hipSuccess) return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
+  if ((e = grow(dv, sizeof(double) * chk_val.size())) != hipSuccess) return set_err(h, KIFMM_ERR_HIP, hipGetErrorString(e));
   int rc = f.direct(dd.as<double>(), int(idx.size()), dp.as<double>(), dv.as<double>(), nullptr);
   if (rc) return rc;
   if ((e = hipMemcpy(chk_val.data(), dv.as<double>(), sizeof(double) * chk_val.size(), hipMemcpyDeviceToHost)) !=

@@ -611,10 +611,12 @@ int FMM3d::upload_state() {
   if ((rc = check((buf).upload((vec).data(), sizeof((vec)[0]) * (vec).size()), what))) return rc;
   // packed points (leaf order)
   std::vector<double> sp(3 * ns_), sn(3 * ns_), tp(3 * nt_);
+#pragma omp parallel for schedule(static)
   for (int64_t k = 0; k < ns_; k++) {
     int64_t o = t.src_perm[k];
     for (int d = 0; d < 3; d++) { sp[3 * k + d] = spos_[3 * o + d]; sn[3 * k + d] = snor_[3 * o + d]; }
   }
+#pragma omp parallel for schedule(static)
   for (int64_t k = 0; k < nt_; k++) {
     int64_t o = t.trg_perm[k];
     for (int d = 0; d < 3; d++) tp[3 * k + d] = tpos_[3 * o + d];
@@ -628,9 +630,12 @@ int FMM3d::upload_state() {
     // order and each value is written once (random reads of 8 bytes cost a
     // whole memory burst each)
     std::vector<int> inv(size_t(std::max<int64_t>(1, ns_)), -1);
+#pragma omp parallel for schedule(static)
     for (int64_t k = 0; k < ns_; k++) inv[size_t(t.src_perm[size_t(k)])] = int(k);
-    for (int64_t k = 0; k < ns_; k++)
-      if (inv[size_t(k)] < 0) return fail(-6, "source permutation is not a permutation");
+    int64_t holes = 0;
+#pragma omp parallel for schedule(static) reduction(+ : holes)
+    for (int64_t k = 0; k < ns_; k++) holes += inv[size_t(k)] < 0;
+    if (holes) return fail(-6, "source permutation is not a permutation");
     KIFMM_UP(sinv_d_, inv, "upload source permutation");
   }
   KIFMM_UP(tperm_d_, t.trg_perm, "upload target permutation");
@@ -638,6 +643,7 @@ int FMM3d::upload_state() {
   if ((rc = check(tval_d_.alloc(sizeof(double) * std::max<int64_t>(1, nt_ * TD)), "alloc values"))) return rc;
   // node geometry
   std::vector<double4> geom(N);
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < N; i++) {
     double c[3];
     t.center(i, c);

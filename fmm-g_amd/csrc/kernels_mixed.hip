@@ -827,7 +827,7 @@ __global__ __launch_bounds__(64) void k_m2l_tile(VlistArgs v, const float2* __re
 // translation value feeds 8 G complex multiply-adds (the per-pair kernel
 // k_hadamard_mx reads one per multiply-add).  Pairs of adjacent children fall
 // on the zero spectra of the table's |offset| <= 1 slots, as in k_m2l_tile.
-template <int SD, int TD, int G>
+template <int SD, int TD, int G, bool BUFX>
 __global__ __launch_bounds__(64) void k_m2l_tile_vec(VlistArgs v, const float2* __restrict__ dspec,
                                                      const float2* __restrict__ m2l, float2* __restrict__ espec,
                                                      float nrm, int ntile, int per_xcd) {
@@ -840,6 +840,27 @@ __global__ __launch_bounds__(64) void k_m2l_tile_vec(VlistArgs v, const float2* 
   const int fs = fok ? f : H - 1;
   const int g0 = tile * G;
   const int ng = min(G, v.ngrp * v.nrhs - g0);
+  // spectra rows through buffer resources: the row offset (uniform) in the
+  // scalar offset, the lane's frequency in the vector offset, so no load needs
+  // vector address arithmetic (the translation table always fits 32-bit
+  // offsets; the source spectra when BUFX)
+  const unsigned H8 = unsigned(H) * 8u, fs8 = unsigned(fs) * 8u;
+  const __amdgpu_buffer_rsrc_t rs_t =
+      __builtin_amdgcn_make_buffer_rsrc((void*)m2l, (short)0, int(343u * SD * TD * H8), 0x00020000);
+  const __amdgpu_buffer_rsrc_t rs_x = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)dspec, (short)0, int(unsigned(v.nsrc) * unsigned(v.nrhs) * SD * H8), 0x00020000);
+  auto ldt = [&](int row) -> float2 {
+    const tf2 t = __builtin_bit_cast(tf2, __builtin_amdgcn_raw_buffer_load_b64(rs_t, fs8, unsigned(row) * H8, 0));
+    return make_float2(t.x, t.y);
+  };
+  auto ldx = [&](int row) -> float2 {
+    if constexpr (BUFX) {
+      const tf2 x = __builtin_bit_cast(tf2, __builtin_amdgcn_raw_buffer_load_b64(rs_x, fs8, unsigned(row) * H8, 0));
+      return make_float2(x.x, x.y);
+    } else {
+      return dspec[int64_t(row) * H + fs];
+    }
+  };
   __shared__ int shal[G * 216];
   for (int i = threadIdx.x; i < G * 216; i += 64) {
     int h = -1;
@@ -863,42 +884,38 @@ __global__ __launch_bounds__(64) void k_m2l_tile_vec(VlistArgs v, const float2* 
     const int qx = q / 9 - 1, qy = (q / 3) % 3 - 1, qz = q % 3 - 1;
     const int hbase = (2 * qx + 2) * 36 + (2 * qy + 2) * 6 + (2 * qz + 2);
     const int sbase = (2 * qx + 3) * 49 + (2 * qy + 3) * 7 + (2 * qz + 3);
-    int si[G][8];
-    // translation slots (of the 27) that a present source needs: source child c
-    // meets the 2 x 2 x 2 block of offsets c - tau + 1
-    unsigned need = 0;
+    // a bit per present (group, source child) of this colleague
+    unsigned pres = 0;
 #pragma unroll
     for (int gi = 0; gi < G; gi++)
 #pragma unroll
-      for (int c = 0; c < 8; c++) {
-        si[gi][c] = __builtin_amdgcn_readfirstlane(
-            shal[gi * 216 + hbase + (c >> 2) * 36 + ((c >> 1) & 1) * 6 + (c & 1)]);
-        if (si[gi][c] >= 0) {
-          const int b = (c >> 2) * 9 + ((c >> 1) & 1) * 3 + (c & 1);
-          need |= 0x361bu << b;  // offsets {0,1} x {0,1} x {0,1} in base-3 digits from b
-        }
-      }
-    if (!need) continue;  // no source in this colleague: nothing to load
+      for (int c = 0; c < 8; c++)
+        if (shal[gi * 216 + hbase + (c >> 2) * 36 + ((c >> 1) & 1) * 6 + (c & 1)] >= 0) pres |= 1u << (gi * 8 + c);
+    pres = __builtin_amdgcn_readfirstlane(pres);
+    if (!pres) continue;  // no source in this colleague
     for (int j = 0; j < SD; j++) {
       float2 x[G][8];
 #pragma unroll
       for (int gi = 0; gi < G; gi++)
 #pragma unroll
-        for (int c = 0; c < 8; c++)
-          x[gi][c] = dspec[(int64_t(si[gi][c] < 0 ? 0 : si[gi][c]) * SD + j) * H + fs];
+        for (int c = 0; c < 8; c++) {
+          const int sc = __builtin_amdgcn_readfirstlane(
+              shal[gi * 216 + hbase + (c >> 2) * 36 + ((c >> 1) & 1) * 6 + (c & 1)]);
+          x[gi][c] = ldx((sc < 0 ? 0 : sc) * SD + j);
+        }
 #pragma unroll
       for (int i = 0; i < TD; i++) {
         float2 T[27];
 #pragma unroll
         for (int d = 0; d < 27; d++) {
           const int sl = sbase + (d / 9 - 1) * 49 + ((d / 3) % 3 - 1) * 7 + (d % 3 - 1);
-          T[d] = (need >> d) & 1 ? m2l[(int64_t(sl) * SD * TD + j * TD + i) * H + fs] : make_float2(0.f, 0.f);
+          T[d] = ldt(sl * (SD * TD) + j * TD + i);
         }
 #pragma unroll
         for (int gi = 0; gi < G; gi++)
 #pragma unroll
           for (int c = 0; c < 8; c++) {
-            if (si[gi][c] < 0) continue;
+            if (!((pres >> (gi * 8 + c)) & 1)) continue;
             const float2 xv = x[gi][c];
 #pragma unroll
             for (int t = 0; t < 8; t++) {
@@ -1061,16 +1078,22 @@ hipError_t launch_hadamard_mixed(const VlistArgs& v, const float2* dspec, const 
     const int ntile = int((nvg + G - 1) / G);
     const int per_xcd = (ntile + 7) / 8;
     const dim3 grid2(unsigned(8 * per_xcd), unsigned((v.H + 63) / 64));
+    // source spectra through a buffer resource when they fit 32-bit offsets
+    const bool bufx = uint64_t(v.nsrc) * v.nrhs * v.sd * v.H * 8 <= 0xFFFFFF00ull;
+#define KIFMM_TILE_G(SD_, TD_, G_)                                                                          \
+  if (bufx) k_m2l_tile_vec<SD_, TD_, G_, true><<<grid2, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm, ntile, per_xcd); \
+  else k_m2l_tile_vec<SD_, TD_, G_, false><<<grid2, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm, ntile, per_xcd);
 #define KIFMM_TILE(SD_, TD_)                                                                                \
   if (v.sd == SD_ && v.td == TD_) {                                                                         \
-    if (G == 1) k_m2l_tile_vec<SD_, TD_, 1><<<grid2, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm, ntile, per_xcd); \
-    else k_m2l_tile_vec<SD_, TD_, 2><<<grid2, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm, ntile, per_xcd);        \
+    if (G == 1) { KIFMM_TILE_G(SD_, TD_, 1) }                                                               \
+    else { KIFMM_TILE_G(SD_, TD_, 2) }                                                                      \
     return hipGetLastError();                                                                               \
   }
     KIFMM_TILE(4, 3)
     KIFMM_TILE(3, 3)
     KIFMM_TILE(3, 4)
 #undef KIFMM_TILE
+#undef KIFMM_TILE_G
     return hipErrorInvalidValue;
   }
   return hipGetLastError();

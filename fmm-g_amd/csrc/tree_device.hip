@@ -157,13 +157,13 @@ struct SplitDesc {
   int depth, prob;
 };
 
-__global__ void k_split_roots(const u128* cells, int64_t m, const int* base, TreeConst c, SplitDesc* desc,
+__global__ void k_split_roots(const u128* cells, int64_t m, const int64_t* base, TreeConst c, SplitDesc* desc,
                               int* groot) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= c.nprob) return;
   const u128 key = u128(p) << (3 * c.L);
   const int64_t f = lower_bound_dev(cells, m, key);
-  const int g = p + base[f];
+  const int g = int(p + base[f]);
   groot[p] = g;
   desc[g].pkey = key;
   desc[g].depth = 0;
@@ -171,14 +171,14 @@ __global__ void k_split_roots(const u128* cells, int64_t m, const int* base, Tre
 }
 
 __global__ void k_split_expand(const u128* cells, int64_t m, const unsigned char* lcp_prev, const int* cnt,
-                               const int* base, TreeConst c, SplitDesc* desc) {
+                               const int64_t* base, TreeConst c, SplitDesc* desc) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= m || cnt[i] == 0) return;
   const u128 key = cells[i];
   const int p = int(key >> (3 * c.L));
   for (int j = 0; j < cnt[i]; j++) {
     const int d = lcp_prev[i] + 1 + j;
-    const int g = p + 1 + base[i] + j;
+    const int g = int(p + 1 + base[i] + j);
     const u128 low = (u128(1) << (3 * (c.L - d))) - 1;
     desc[g].pkey = key & ~low;
     desc[g].depth = d;
@@ -187,13 +187,13 @@ __global__ void k_split_expand(const u128* cells, int64_t m, const unsigned char
 }
 
 // global rank of the split node that starts at cell j with depth d (d >= 1)
-__device__ inline int rank_at(int p, int64_t j, int d, const int* base, const unsigned char* lcp_prev) {
-  return p + 1 + base[j] + (d - lcp_prev[j] - 1);
+__device__ inline int rank_at(int p, int64_t j, int d, const int64_t* base, const unsigned char* lcp_prev) {
+  return int(p + 1 + base[j] + (d - lcp_prev[j] - 1));
 }
 
 // Node records of every split node and of its 8 children (CreateAncestors:
 // child block at p + 1 + 8g, child 4x+2y+z at path 2 path + (x, y, z)).
-__global__ void k_split_nodes(const SplitDesc* desc, int G, const u128* cells, int64_t m, const int* base,
+__global__ void k_split_nodes(const SplitDesc* desc, int G, const u128* cells, int64_t m, const int64_t* base,
                               const unsigned char* lcp_prev, const int* groot, TreeConst c, int* snode,
                               int* parent, int* child, int* depth, int* path, int* proot) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -276,7 +276,7 @@ __global__ void k_leaf_ranges(const int* sorted_leaf, int64_t n, int* beg) {
 
 // Subtree point counts: leaves their own; split node g the sum over the node
 // range of its descendants [p+1+8g, p+1+8 g_end).
-__global__ void k_subtree_counts(const SplitDesc* desc, int G, const u128* cells, int64_t m, const int* base,
+__global__ void k_subtree_counts(const SplitDesc* desc, int G, const u128* cells, int64_t m, const int64_t* base,
                                  const int* snode, TreeConst c, const int64_t* spre, const int64_t* tpre,
                                  int* scnt, int* tcnt) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -284,7 +284,7 @@ __global__ void k_subtree_counts(const SplitDesc* desc, int G, const u128* cells
   const SplitDesc s = desc[g];
   const u128 span = u128(1) << (3 * (c.L - s.depth));
   const int64_t e = lower_bound_dev(cells, m, s.pkey + span);
-  const int gend = s.prob + 1 + base[e];
+  const int gend = int(s.prob + 1 + base[e]);
   const int64_t a = s.prob + 1 + 8 * int64_t(g), b = s.prob + 1 + 8 * int64_t(gend);
   scnt[snode[g]] = int(spre[b] - spre[a]);
   tcnt[snode[g]] = int(tpre[b] - tpre[a]);
@@ -611,7 +611,8 @@ int build_tree_device(int nprob, const int64_t* ns, const double* spos, const in
 
   // ---- split nodes ----
   unsigned char* d_lcp;
-  int *d_cnt, *d_base;
+  int* d_cnt;
+  int64_t* d_base;  // split nodes before each cell: 64-bit, checked before the node arrays
   TRY(pool.get(&d_lcp, M));
   TRY(pool.get(&d_cnt, M + 1));
   TRY(pool.get(&d_base, M + 1));
@@ -620,13 +621,13 @@ int build_tree_device(int nprob, const int64_t* ns, const double* spos, const in
   TRY(hipGetLastError());
   {
     size_t tb = 0;
-    TRY(rocprim::exclusive_scan(nullptr, tb, d_cnt, d_base, 0, size_t(M + 1), rocprim::plus<int>(), s));
+    TRY(rocprim::exclusive_scan(nullptr, tb, d_cnt, d_base, int64_t(0), size_t(M + 1), rocprim::plus<int64_t>(), s));
     char* tmp;
     TRY(pool.get(&tmp, int64_t(tb)));
-    TRY(rocprim::exclusive_scan(tmp, tb, d_cnt, d_base, 0, size_t(M + 1), rocprim::plus<int>(), s));
+    TRY(rocprim::exclusive_scan(tmp, tb, d_cnt, d_base, int64_t(0), size_t(M + 1), rocprim::plus<int64_t>(), s));
   }
-  int S = 0;
-  TRY(hipMemcpyAsync(&S, d_base + M, 4, hipMemcpyDeviceToHost, s));
+  int64_t S = 0;
+  TRY(hipMemcpyAsync(&S, d_base + M, 8, hipMemcpyDeviceToHost, s));
   TRY(hipStreamSynchronize(s));
   const int64_t G = int64_t(S) + nprob, N64 = nprob + 8 * G;
   if (N64 > INT32_MAX / 2) { if (err) *err = "tree too large for 32-bit node indices"; return -2; }
