@@ -237,14 +237,17 @@ __global__ __launch_bounds__(kNearChunk) void k_near_mx(NearArgs a, KParams<floa
 // Laplace single layer (user and equivalent-density kernel 111).  The sources of
 // a chunk's items are packed, item after item, into blocks of kNearChunk float4
 // (x, y, z relative to the leaf centre, density / 4 pi): a lane finds its item
-// by binary search over an LDS prefix of the item sizes (a wave-shuffle scan),
-// so the ~50 small items of a leaf (37 points on average) share a few staging
-// rounds.  Exact and UE items are packed together (near part), the DE item (last
-// in the list) separately (far part).  One broadcast LDS read per pair; lanes
-// form S copies of the chunk's targets and each copy sweeps a contiguous share
-// of the block (source count rounded up to 4, split S ways in multiples of 4),
-// unrolled by 4.  Padding entries have zero weight and lie 1e18 away, so they
-// add exact zeros.
+// from an LDS prefix of the item sizes (a wave-shuffle scan), walking the few
+// items that start inside the block in a uniform loop, so the ~50 small items
+// of a leaf (37 points on average) share a few staging rounds.  Exact and UE
+// items are packed together (near part), the DE item (last in the list)
+// separately (far part).  One broadcast LDS read per pair; lanes form S copies
+// of the chunk's targets and each copy sweeps a contiguous share of the block
+// (source count rounded up to 4, split S ways in multiples of 4), unrolled by 4
+// with the same (uniform) trip count for every copy: the shares of the last
+// copies run on into padding, which the block buffer carries up to kNearPad
+// entries.  Padding entries have zero weight and lie 1e18 away, so they add
+// exact zeros.
 // Pairs closer than the reference's 1e-8 cut contribute nothing.  Only a
 // source of the target leaf itself can come that close, unless the host found
 // another item's sources within 1e-7 of the chunk's targets (then unchk = 0):
@@ -252,13 +255,13 @@ __global__ __launch_bounds__(kNearChunk) void k_near_mx(NearArgs a, KParams<floa
 // the bare v_rsq_f32: r^2 >= 1e-16 there, a normal float).  The FP32 partial
 // sums per block are added in FP64, near (exact, UE) and far (DE) parts
 // separately.
-__device__ __forceinline__ void lap_sweep(const float4* s4, int j0, int j1, float tx, float ty, float tz,
-                                          float& acc0, float& acc1) {
+__device__ __forceinline__ void lap_sweep(const float4* s4, int n4, float tx, float ty, float tz, float& acc0,
+                                          float& acc1) {
   const float eps2 = float(kMinDif * kMinDif);
-  for (int j = j0; j < j1; j += 4) {
+  for (int i = 0; i < n4; i++, s4 += 4) {
 #pragma unroll
     for (int u = 0; u < 4; u += 2) {
-      const float4 s0 = s4[j + u], s1 = s4[j + u + 1];
+      const float4 s0 = s4[u], s1 = s4[u + 1];
       const float dx0 = tx - s0.x, dy0 = ty - s0.y, dz0 = tz - s0.z;
       const float dx1 = tx - s1.x, dy1 = ty - s1.y, dz1 = tz - s1.z;
       const float r0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
@@ -269,12 +272,12 @@ __device__ __forceinline__ void lap_sweep(const float4* s4, int j0, int j1, floa
   }
 }
 
-__device__ __forceinline__ void lap_sweep_nc(const float4* s4, int j0, int j1, float tx, float ty, float tz,
-                                             float& acc0, float& acc1) {
-  for (int j = j0; j < j1; j += 4) {
+__device__ __forceinline__ void lap_sweep_nc(const float4* s4, int n4, float tx, float ty, float tz, float& acc0,
+                                             float& acc1) {
+  for (int i = 0; i < n4; i++, s4 += 4) {
 #pragma unroll
     for (int u = 0; u < 4; u += 2) {
-      const float4 s0 = s4[j + u], s1 = s4[j + u + 1];
+      const float4 s0 = s4[u], s1 = s4[u + 1];
       const float dx0 = tx - s0.x, dy0 = ty - s0.y, dz0 = tz - s0.z;
       const float dx1 = tx - s1.x, dy1 = ty - s1.y, dz1 = tz - s1.z;
       const float r0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
@@ -290,13 +293,13 @@ __device__ __forceinline__ void lap_sweep_nc(const float4* s4, int j0, int j1, f
 // them.  Each right-hand side accumulates in the order of lap_sweep (even and
 // odd sources in two sums), so its values are those of a one-RHS launch.
 template <int NR, bool CUT>
-__device__ __forceinline__ void lap_sweep_nr(const float4* s4, const float* sw, int j0, int j1, float tx, float ty,
-                                             float tz, float (&acc0)[NR], float (&acc1)[NR]) {
+__device__ __forceinline__ void lap_sweep_nr(const float4* s4, const float* sw, int n4, float tx, float ty, float tz,
+                                             float (&acc0)[NR], float (&acc1)[NR]) {
   const float eps2 = float(kMinDif * kMinDif);
-  for (int j = j0; j < j1; j += 4) {
+  for (int i = 0; i < n4; i++, s4 += 4, sw += 4 * NR) {
 #pragma unroll
     for (int u = 0; u < 4; u += 2) {
-      const float4 s0 = s4[j + u], s1 = s4[j + u + 1];
+      const float4 s0 = s4[u], s1 = s4[u + 1];
       const float dx0 = tx - s0.x, dy0 = ty - s0.y, dz0 = tz - s0.z;
       const float dx1 = tx - s1.x, dy1 = ty - s1.y, dz1 = tz - s1.z;
       const float r0 = dx0 * dx0 + dy0 * dy0 + dz0 * dz0;
@@ -305,18 +308,20 @@ __device__ __forceinline__ void lap_sweep_nr(const float4* s4, const float* sw, 
       const float g1 = CUT ? (r1 < eps2 ? 0.f : rsqrtf(r1)) : __builtin_amdgcn_rsqf(r1);
 #pragma unroll
       for (int r = 0; r < NR; r++) {
-        acc0[r] = fmaf(g0, sw[(j + u) * NR + r], acc0[r]);
-        acc1[r] = fmaf(g1, sw[(j + u + 1) * NR + r], acc1[r]);
+        acc0[r] = fmaf(g0, sw[u * NR + r], acc0[r]);
+        acc1[r] = fmaf(g1, sw[(u + 1) * NR + r], acc1[r]);
       }
     }
   }
 }
 
 constexpr int kItemWin = 128;
+// S copies of L = ceil4(ceil(m4 / S)) sources reach at most m4 + 4 S - 1 < 192
+constexpr int kNearPad = kNearChunk + 64;
 template <int NR>
 struct LapShared {
-  float4 s4[kNearChunk];
-  float sw[NR > 1 ? kNearChunk * NR : 1];
+  float4 s4[kNearPad];
+  float sw[NR > 1 ? kNearPad * NR : 1];
   int pre[kItemWin + 1];
   int ibeg_[kItemWin];
   unsigned char ikind_[kItemWin];
@@ -347,6 +352,13 @@ __device__ __forceinline__ void near_lap_chunk(const NearArgs a, int chunk, floa
   if (active) {
     const double* p = a.tpos + 3 * int64_t(ch.tbeg + ti);
     tx = float(p[0] - cb.x); ty = float(p[1] - cb.y); tz = float(p[2] - cb.z);
+  }
+  // padding past the block (read by the last copies' shares, never restaged)
+  if (lane < kNearPad - kNearChunk) {
+    s4[kNearChunk + lane] = make_float4(1e18f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+      if (NR > 1) sw[(kNearChunk + lane) * NR + r] = 0.f;
   }
   const int idiv = ch.iend > ch.ibeg && (a.items[ch.iend - 1] & 3) == ITEM_DE ? ch.iend - 1 : ch.iend;
   // the near (exact, UE) and far (DE) sums; each pass accumulates into pp and
@@ -386,9 +398,23 @@ __device__ __forceinline__ void near_lap_chunk(const NearArgs a, int chunk, floa
       // slots of the target leaf's own item in this window (empty if absent)
       const int ow = ch.own >= 0 ? ch.ibeg + ch.own - w0 : -1;
       const int own0 = ow >= 0 && ow < nw ? pre[ow] : 0, own1 = ow >= 0 && ow < nw ? pre[ow + 1] : 0;
+      // icur (uniform): the item of slot `base`, the last j with pre[j] <= base
+      int icur = 0;
       for (int base = 0; base < total; base += kNearChunk) {
         const int m = min(kNearChunk, total - base);
         const bool cut = !ch.unchk || (own0 < base + m && own1 > base);
+        // item of slot base + lane: the last j >= icur with pre[j] <= base + lane.
+        // The items starting inside the block (2.7 on average for leaves of
+        // ~40 points) are walked in a uniform loop.
+        int lo = icur, j = icur + 1;
+        for (; j < nw; j++) {
+          const int sj = __builtin_amdgcn_readfirstlane(pre[j]);
+          if (sj >= base + m) break;
+          if (base + lane >= sj) lo = j;
+        }
+        int inext = j - 1;
+        while (inext + 1 < nw && __builtin_amdgcn_readfirstlane(pre[inext + 1]) <= base + kNearChunk) inext++;
+        icur = inext;
         __syncthreads();
         float4 v = make_float4(1e18f, 0.f, 0.f, 0.f);
         float wv[NR];
@@ -396,12 +422,6 @@ __device__ __forceinline__ void near_lap_chunk(const NearArgs a, int chunk, floa
         for (int r = 0; r < NR; r++) wv[r] = 0.f;
         if (lane < m) {
           const int p = base + lane;
-          int lo = 0, hi = nw - 1;
-          while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (pre[mid] <= p) lo = mid;
-            else hi = mid - 1;
-          }
           const int off = p - pre[lo], kind = ikind_[lo];
           if (kind == ITEM_EXACT) {
             const int64_t q = int64_t(ibeg_[lo] + off);
@@ -434,18 +454,18 @@ __device__ __forceinline__ void near_lap_chunk(const NearArgs a, int chunk, floa
         if (active) {
           const int m4 = (m + 3) & ~3;
           const int L = ((m4 + S - 1) / S + 3) & ~3;
-          const int j0 = sub * L, j1 = min(j0 + L, m4);
+          const int j0 = sub * L, n4 = L >> 2;
           if constexpr (NR == 1) {
             float acc0 = 0.f, acc1 = 0.f;
-            if (cut) lap_sweep(s4, j0, j1, tx, ty, tz, acc0, acc1);
-            else lap_sweep_nc(s4, j0, j1, tx, ty, tz, acc0, acc1);
+            if (cut) lap_sweep(s4 + j0, n4, tx, ty, tz, acc0, acc1);
+            else lap_sweep_nc(s4 + j0, n4, tx, ty, tz, acc0, acc1);
             pp[0] += double(acc0 + acc1);
           } else {
             float acc0[NR], acc1[NR];
 #pragma unroll
             for (int r = 0; r < NR; r++) acc0[r] = acc1[r] = 0.f;
-            if (cut) lap_sweep_nr<NR, true>(s4, sw, j0, j1, tx, ty, tz, acc0, acc1);
-            else lap_sweep_nr<NR, false>(s4, sw, j0, j1, tx, ty, tz, acc0, acc1);
+            if (cut) lap_sweep_nr<NR, true>(s4 + j0, sw + j0 * NR, n4, tx, ty, tz, acc0, acc1);
+            else lap_sweep_nr<NR, false>(s4 + j0, sw + j0 * NR, n4, tx, ty, tz, acc0, acc1);
 #pragma unroll
             for (int r = 0; r < NR; r++) pp[r] += double(acc0[r] + acc1[r]);
           }
