@@ -316,6 +316,17 @@ __device__ __forceinline__ void lap_sweep_nr(const float4* s4, const float* sw, 
 }
 
 constexpr int kItemWin = 128;
+typedef const __attribute__((address_space(1))) double* gdptr;
+// An item of the staging window: slot j of the item lies at
+// (cw[0..2]) + cw[3] * pos[3j .. 3j + 2] relative to the leaf centre (exact
+// items: (-leaf centre, 1) and their source points, surface items: (node
+// centre - leaf centre, radius) and the surface samples); den is its first
+// density.
+struct NearItem {
+  double cw[4];
+  gdptr pos;
+  gdptr den;
+};
 // S copies of L = ceil4(ceil(m4 / S)) sources reach at most m4 + 4 S - 1 < 192
 constexpr int kNearPad = kNearChunk + 64;
 template <int NR>
@@ -323,10 +334,12 @@ struct LapShared {
   float4 s4[kNearPad];
   float sw[NR > 1 ? kNearPad * NR : 1];
   int pre[kItemWin + 1];
-  int ibeg_[kItemWin];
   unsigned char ikind_[kItemWin];
   int wsum[2];
-  double red[kNearChunk * 2];
+  union {
+    NearItem irec_[kItemWin];
+    double red[kNearChunk * 2];  // the final reduction (after the last window)
+  };
   int next;  // work queue: the chunk this workgroup takes next
 };
 
@@ -336,7 +349,7 @@ __device__ __forceinline__ void near_lap_chunk(const NearArgs a, int chunk, floa
   float4* s4 = sh.s4;
   float* sw = sh.sw;
   int* pre = sh.pre;
-  int* ibeg_ = sh.ibeg_;
+  NearItem* irec_ = sh.irec_;
   unsigned char* ikind_ = sh.ikind_;
   int* wsum = sh.wsum;
   double* red = sh.red;
@@ -360,6 +373,7 @@ __device__ __forceinline__ void near_lap_chunk(const NearArgs a, int chunk, floa
     for (int r = 0; r < NR; r++)
       if (NR > 1) sw[(kNearChunk + lane) * NR + r] = 0.f;
   }
+  const int64_t rs_ue = a.nstride * a.sz_ue, rs_de = a.nstride * a.sz_de;  // right-hand-side steps
   const int idiv = ch.iend > ch.ibeg && (a.items[ch.iend - 1] & 3) == ITEM_DE ? ch.iend - 1 : ch.iend;
   // the near (exact, UE) and far (DE) sums; each pass accumulates into pp and
   // hands it over at its end (static indices: no scratch for a dynamic pass)
@@ -377,9 +391,21 @@ __device__ __forceinline__ void near_lap_chunk(const NearArgs a, int chunk, floa
       int sz = 0;
       if (lane < nw) {
         const int code = a.items[w0 + lane], node = code >> 2, kind = code & 3;
-        const bool ex = kind == ITEM_EXACT;
-        sz = ex ? a.src_num[node] : (kind == ITEM_UE ? a.n_ue : a.n_de);
-        ibeg_[lane] = ex ? a.src_beg[node] : node;
+        const bool ex = kind == ITEM_EXACT, ue = kind == ITEM_UE;
+        sz = ex ? a.src_num[node] : (ue ? a.n_ue : a.n_de);
+        NearItem& it = irec_[lane];
+        if (ex) {
+          const int64_t b = a.src_beg[node];
+          it.pos = (gdptr)(a.spos + 3 * b);
+          it.den = (gdptr)(a.sden + (b * a.nrhs + a.r0));
+          it.cw[0] = -cb.x; it.cw[1] = -cb.y; it.cw[2] = -cb.z; it.cw[3] = 1.0;
+        } else {
+          const double4 g = a.geom[node];
+          it.pos = (gdptr)(ue ? a.sam_ue : a.sam_de);
+          it.den = (gdptr)(ue ? a.UE + int64_t(node) * a.sz_ue + a.r0 * rs_ue
+                              : a.DE + int64_t(node) * a.sz_de + a.r0 * rs_de);
+          it.cw[0] = g.x - cb.x; it.cw[1] = g.y - cb.y; it.cw[2] = g.z - cb.z; it.cw[3] = g.w;
+        }
         ikind_[lane] = (unsigned char)kind;
       }
       int inc = sz;
@@ -422,26 +448,21 @@ __device__ __forceinline__ void near_lap_chunk(const NearArgs a, int chunk, floa
         for (int r = 0; r < NR; r++) wv[r] = 0.f;
         if (lane < m) {
           const int p = base + lane;
-          const int off = p - pre[lo], kind = ikind_[lo];
-          if (kind == ITEM_EXACT) {
-            const int64_t q = int64_t(ibeg_[lo] + off);
-            v = make_float4(float(a.spos[3 * q] - cb.x), float(a.spos[3 * q + 1] - cb.y),
-                            float(a.spos[3 * q + 2] - cb.z), 0.f);
+          // one branch-free form for both kinds: exact items carry
+          // (-centre, 1), surface items (node centre - leaf centre, radius)
+          const unsigned off = unsigned(p - pre[lo]);
+          const int kind = ikind_[lo];
+          const bool ex = kind == ITEM_EXACT;
+          const NearItem& it = irec_[lo];
+          const gdptr ps = it.pos + ((off << 1) + off);
+          const double w = it.cw[3];
+          v = make_float4(float(fma(w, ps[0], it.cw[0])), float(fma(w, ps[1], it.cw[1])),
+                          float(fma(w, ps[2], it.cw[2])), 0.f);
+          const gdptr pd = it.den + (ex ? unsigned(__mul24(int(off), a.nrhs)) : off);
+          const float sc = ex ? pa_u : pa_m;
 #pragma unroll
-            for (int r = 0; r < NR; r++)
-              if (r < a.nr) wv[r] = float(a.sden[q * a.nrhs + a.r0 + r]) * pa_u;
-          } else {
-            const int node = ibeg_[lo];
-            const double4 g = a.geom[node];
-            const double* sam = kind == ITEM_UE ? a.sam_ue : a.sam_de;
-            const double* den = kind == ITEM_UE ? a.UE + int64_t(node) * a.sz_ue : a.DE + int64_t(node) * a.sz_de;
-            const int64_t rstep = a.nstride * (kind == ITEM_UE ? a.sz_ue : a.sz_de);
-            v = make_float4(float((g.x - cb.x) + g.w * sam[3 * off]), float((g.y - cb.y) + g.w * sam[3 * off + 1]),
-                            float((g.z - cb.z) + g.w * sam[3 * off + 2]), 0.f);
-#pragma unroll
-            for (int r = 0; r < NR; r++)
-              if (r < a.nr) wv[r] = float(den[(a.r0 + r) * rstep + off]) * pa_m;
-          }
+          for (int r = 0; r < NR; r++)
+            if (r < a.nr) wv[r] = float(pd[r * (ex ? int64_t(1) : (kind == ITEM_UE ? rs_ue : rs_de))]) * sc;
         }
         if constexpr (NR == 1) {
           v.w = wv[0];
