@@ -1304,8 +1304,11 @@ int FMM3d::autotune(const double* d_src_den, double* d_trg_val, int nrhs, hipStr
     return t;
   };
   struct Knob { int id; std::vector<int> values; };
-  std::vector<Knob> knobs = {{2, {0, 6}}, {1, {0, 1, 2}}, {0, {0, 1, 2}}};
+  // vector kernels: the Hadamard form first (the V list dominates their
+  // evaluate, so the other trials run with the chosen form)
+  std::vector<Knob> knobs;
   if (ops_->sd > 1 || ops_->td > 1) knobs.push_back({3, {0, 1, 2}});
+  knobs.insert(knobs.end(), {{2, {0, 6}}, {1, {0, 1, 2}}, {0, {0, 1, 2}}});
   int rc = 0;
   for (const Knob& k : knobs) {
     Tuning base = tune_;

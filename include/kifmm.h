@@ -195,7 +195,7 @@ int kifmm_set_translation(kifmm_handle h, int terms);
  * reference has no launch shapes to choose).  Times each choice on this
  * handle's own tree with the caller's device densities (nrhs right-hand sides,
  * as kifmm_evaluate_multi_device; d_trg_val receives values) and keeps the
- * fastest, one choice at a time:
+ * fastest, one choice at a time, in the order 3 (vector kernels only), 2, 1, 0:
  *   knob 2: translations, 0 (FP64 matrix cores, fused M2M) or 6 (split BF16x6),
  *           timed over the whole evaluate;
  *   knob 1: staging weight of the near-field chunk split, index 0..2 of
