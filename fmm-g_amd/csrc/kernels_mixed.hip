@@ -458,7 +458,7 @@ __device__ __forceinline__ void near_lap_chunk(const NearArgs a, int chunk, floa
           const double w = it.cw[3];
           v = make_float4(float(fma(w, ps[0], it.cw[0])), float(fma(w, ps[1], it.cw[1])),
                           float(fma(w, ps[2], it.cw[2])), 0.f);
-          const gdptr pd = it.den + (ex ? unsigned(__mul24(int(off), a.nrhs)) : off);
+          const gdptr pd = it.den + (ex ? off * unsigned(a.nrhs) : off);
           const float sc = ex ? pa_u : pa_m;
 #pragma unroll
           for (int r = 0; r < NR; r++)
