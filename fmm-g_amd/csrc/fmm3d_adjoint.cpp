@@ -348,7 +348,28 @@ int FMM3d::evaluateAdjoint(const double* d_trg_in, double* d_src_out, hipStream_
   na.sam_uc = sam_d_[UC].as<double>(); na.n_uc = op.n_sam[UC]; na.UC = uc_d_.as<double>(); na.sz_uc = op.dat(UC);
   na.sout = sden_d_.as<double>();
   if ((rc = check(launch_fill(sden_d_.as<double>(), std::max<int64_t>(1, ns_ * SD), 0.0, s), "zero output"))) return rc;
-  if ((rc = check(launch_near_adjoint(kt_, na, pu, s), "near field adjoint"))) return rc;
+  if (precision_ == PREC_MIXED && kt_ == 111) {
+    // Laplace is symmetric, so the transposed pairs are the forward mixed near
+    // kernel's with the roles swapped: the chunk's points are the sources, the
+    // exact items the target leaves (target points weighted by the adjoint
+    // input), and the two surface kinds the S2L check surfaces (DCbar) and the
+    // leaf's own S2M check surface (UCbar).  The item codes ADJ_TRG, ADJ_DC and
+    // ADJ_UC are ITEM_EXACT, ITEM_UE and ITEM_DE.
+    static_assert(int(ADJ_TRG) == int(ITEM_EXACT) && int(ADJ_DC) == int(ITEM_UE) && int(ADJ_UC) == int(ITEM_DE),
+                  "adjoint item kinds must match the near-kernel item kinds");
+    NearArgs a{};
+    a.chunks = na.chunks; a.nchunks = na.nchunks; a.items = na.items;
+    a.spos = na.tpos; a.snor = nullptr; a.sden = na.tw;
+    a.src_beg = na.trg_beg; a.src_num = na.trg_num;
+    a.geom = na.geom;
+    a.sam_ue = na.sam_dc; a.n_ue = na.n_dc; a.UE = na.DC; a.sz_ue = na.sz_dc;
+    a.sam_de = na.sam_uc; a.n_de = na.n_uc; a.DE = na.UC; a.sz_de = na.sz_uc;
+    a.tpos = na.spos; a.tval = na.sout; a.tperm = nullptr;
+    a.nrhs = 1; a.r0 = 0; a.nr = 1; a.nstride = N; a.tstride = 0;
+    if ((rc = check(launch_near_mixed(kt_, a, pu, pu, s), "near field adjoint"))) return rc;
+  } else if ((rc = check(launch_near_adjoint(kt_, na, pu, s), "near field adjoint"))) {
+    return rc;
+  }
   return check(launch_scatter(sden_d_.as<double>(), sperm_d_.as<int>(), d_src_out, ns_, SD, s), "scatter");
 }
 

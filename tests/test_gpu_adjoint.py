@@ -77,6 +77,38 @@ def test_adjoint_duality_mixed(kt, coefs):
     assert dual_gap(f, 4) < 1e-5
 
 
+@pytest.mark.parametrize("case", ["gaussians", "spheres", "separate"])
+def test_adjoint_mixed_vs_fp64(case):
+    """Laplace mixed transposed pass (FP32 near field through the forward near
+    kernel with the roles of sources and targets swapped) against the FP64
+    transposed pass on the same tree: U, W and X lists active (sphere surfaces),
+    and sources distinct from targets (the swap must read target points for the
+    pairs and source points for the chunks).  Bar: the mixed forward pass's own
+    difference from FP64 on the same points (the FP32 near field sets both; on
+    sphere surfaces it is 2e-6, on Gaussian clusters 1e-7)."""
+    print("case adjoint mixed vs fp64 %s" % case, flush=True)
+    src, trg, nor = None, None, None
+    if case == "gaussians":
+        src = W.gaussians(20000, 61)
+    elif case == "spheres":
+        src, nor = W.spheres(20000, 62, nsph=6)
+    else:
+        src, trg = W.uniform(12000, 63), W.gaussians(9000, 64)
+    fwd, adj = {}, {}
+    for prec in ("fp64", "mixed"):
+        f = kifmm.FMM3d(111, [1.0, 0.25], 6, 32, 30, precision=prec)
+        f.set_points(src, nor, trg)
+        f.setup()
+        rng = np.random.default_rng(65)
+        fwd[prec] = f.evaluate(rng.standard_normal(f.n_src))
+        adj[prec] = f.evaluate_adjoint(rng.standard_normal(f.n_trg))
+        f.close()
+    rf = np.linalg.norm(fwd["mixed"] - fwd["fp64"]) / np.linalg.norm(fwd["fp64"])
+    ra = np.linalg.norm(adj["mixed"] - adj["fp64"]) / np.linalg.norm(adj["fp64"])
+    print("relative difference: forward %.3e, transposed %.3e" % (rf, ra), flush=True)
+    assert ra < 1.5 * rf + 1e-7 and ra < 1e-5
+
+
 @pytest.mark.parametrize("np_,bar", [(4, 1e-12), (6, 1e-10)])
 def test_adjoint_dense_vs_oracle(np_, bar):
     """A formed from the oracle's evaluate, one unit density at a time.  At np 6
