@@ -109,6 +109,34 @@ def test_adjoint_mixed_vs_fp64(case):
     assert ra < 1.5 * rf + 1e-7 and ra < 1e-5
 
 
+@pytest.mark.parametrize("case", ["gaussians", "spheres"])
+def test_field_mixed_vs_fp64(case):
+    """Laplace mixed field pass against the FP64 field pass on the same tree.
+    The mixed field keeps its near field in FP64 (an FP32 gradient near field
+    lost two orders of accuracy on sphere surfaces, where close pairs dominate
+    the 1/r^2 terms), so only its far field is mixed: well inside the mixed
+    forward pass's own difference from FP64."""
+    print("case field mixed vs fp64 %s" % case, flush=True)
+    nor = None
+    if case == "gaussians":
+        src = W.gaussians(20000, 66)
+    else:
+        src, nor = W.spheres(20000, 67, nsph=6)
+    fwd, fld = {}, {}
+    for prec in ("fp64", "mixed"):
+        f = kifmm.FMM3d(111, [1.0, 0.25], 6, 32, 30, precision=prec)
+        f.set_points(src, nor)
+        f.setup()
+        den = W.densities(f.n_src, 1, 68)
+        fwd[prec] = f.evaluate(den)
+        fld[prec] = f.field(den)
+        f.close()
+    rf = np.linalg.norm(fwd["mixed"] - fwd["fp64"]) / np.linalg.norm(fwd["fp64"])
+    rg = np.linalg.norm(fld["mixed"] - fld["fp64"]) / np.linalg.norm(fld["fp64"])
+    print("relative difference: forward %.3e, field %.3e" % (rf, rg), flush=True)
+    assert rg < 1e-6 and rg < rf
+
+
 @pytest.mark.parametrize("np_,bar", [(4, 1e-12), (6, 1e-10)])
 def test_adjoint_dense_vs_oracle(np_, bar):
     """A formed from the oracle's evaluate, one unit density at a time.  At np 6
