@@ -548,9 +548,15 @@ template <int KT, int NR>
 __global__ __launch_bounds__(kSurfMx) void k_surface_mx(SurfArgs a, KParams<float> pu) {
   constexpr int SD = src_dof(KT), TD = trg_dof(KT);
   constexpr bool NOR = needs_normals_ct(KT);
-  __shared__ float sp[kSurfMx * 3];
+  // Laplace, one right-hand side: sources packed as float4 (x, y, z, density)
+  // and swept four at a time (one broadcast LDS read a source, the same
+  // summation order as the scalar loop); blocks padded to a multiple of 4 with
+  // zero-weight points 1e18 away
+  constexpr bool PACK = KT == 111 && NR == 1;
+  __shared__ float sp[PACK ? 1 : kSurfMx * 3];
+  __shared__ float4 s4[PACK ? kSurfMx : 1];
   __shared__ float sn[NOR ? kSurfMx * 3 : 1];
-  __shared__ float sv[kSurfMx * SD * NR];
+  __shared__ float sv[PACK ? 1 : kSurfMx * SD * NR];
   int node, ib, ie, sb1 = 0, sn1 = 0;
   if (a.leaf) {
     const int4 L = a.leaf[blockIdx.x];
@@ -582,7 +588,18 @@ __global__ __launch_bounds__(kSurfMx) void k_surface_mx(SurfArgs a, KParams<floa
     for (int base = 0; base < ns; base += kSurfMx) {
       const int m = min(kSurfMx, ns - base);
       __syncthreads();
-      if (lane < m) {
+      if constexpr (PACK) {
+        if (lane < ((m + 3) & ~3)) {
+          float4 v = make_float4(1e18f, 0.f, 0.f, 0.f);
+          if (lane < m) {
+            const int64_t gg = int64_t(b + base + lane);
+            v = make_float4(float(a.spos[3 * gg] - g.x), float(a.spos[3 * gg + 1] - g.y),
+                            float(a.spos[3 * gg + 2] - g.z),
+                            float(a.sden[gg * a.nrhs + a.r0]) * pu.a);
+          }
+          s4[lane] = v;
+        }
+      } else if (lane < m) {
         const int64_t gg = int64_t(b + base + lane);
         sp[3 * lane] = float(a.spos[3 * gg] - g.x);
         sp[3 * lane + 1] = float(a.spos[3 * gg + 1] - g.y);
@@ -603,7 +620,20 @@ __global__ __launch_bounds__(kSurfMx) void k_surface_mx(SurfArgs a, KParams<floa
 #pragma unroll
         for (int r = 0; r < NR; r++)
           for (int t = 0; t < TD; t++) part[r][t] = 0.0f;
-        if constexpr (KT == 111) {
+        if constexpr (PACK) {
+          // (no pair within the 1e-8 cut, as below)
+          const float tx = tp[u][0], ty = tp[u][1], tz = tp[u][2];
+          float sum = 0.f;
+          for (int j = 0; j < m; j += 4) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+              const float4 q = s4[j + k];
+              const float dx = tx - q.x, dy = ty - q.y, dz = tz - q.z;
+              sum = fmaf(__builtin_amdgcn_rsqf(dx * dx + dy * dy + dz * dz), q.w, sum);
+            }
+          }
+          part[0][0] = sum;
+        } else if constexpr (KT == 111) {
           // check surfaces never meet their sources: the upward check surface
           // lies outside the leaf, and an X-list leaf is not adjacent to the
           // target node whose downward check surface is its boundary, so no
