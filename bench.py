@@ -120,6 +120,13 @@ def cpu_baseline(cfg, pinv, n_sample):
     return n_sample / warm / 1e6, warm, cold
 
 
+def spectrum_len(np_, mixed):
+    """Frequencies per V-list spectrum: the r2c half spectrum (np+1)(2np)^2 in
+    FP64; in the mixed path the packed form, which drops the conjugate mirrors
+    in the planes kc = 0 and kc = np, 4 np^3 + 4."""
+    return 4 * np_ ** 3 + 4 if mixed else (np_ + 1) * (2 * np_) ** 2
+
+
 def phase_roofline(kt, np_, w, phases, mixed, n_src, n_trg):
     """Per-phase roofline: T_roof = max(F / peak, B / HBM) from the work counts of
     the built tree, fraction = T_roof / T_measured, and sum(T_roof) / T_eval.
@@ -137,7 +144,7 @@ def phase_roofline(kt, np_, w, phases, mixed, n_src, n_trg):
     nuc = (np_ + 2) ** 3 - np_ ** 3
     ue, uc, de, dc = nue * sd, nuc * td, nue * sd, nue * td
     G = (2 * np_) ** 3
-    H = (np_ + 1) * (2 * np_) ** 2
+    H = spectrum_len(np_, mixed)
     vec = PEAK_TF["fp32" if mixed else "fp64"] * 1e12
     f64 = PEAK_TF["fp64"] * 1e12
     bw = HBM_GBS * 1e9
@@ -369,7 +376,7 @@ def main():
     # the chip's FP32 peak is the roof.
     had_ms = phases["VListHadamard"]
     wbytes = 4 if mixed else 8
-    H = (np_ + 1) * (2 * np_) ** 2
+    H = spectrum_len(np_, mixed)
     alg_bytes = (w["n_vsrc"] + w["n_vtrg"]) * H * 2 * wbytes + 316 * H * 2 * wbytes
     alg_flops = 8.0 * w["vlist_pairs"] * H
     had_tf = alg_flops / (had_ms * 1e-3) / 1e12
@@ -612,9 +619,9 @@ def bench_batched(R, nprob):
                 res["%s_first_forest_phases_ms" % prec] = {k: round(v, 3) for k, v in ph.items()}
                 f.set_timing(False)
                 # Hadamard roofline: 8 flops per complex multiply-add, 9 components
-                # (sd = td = 3), H = 2304 frequencies at np 8, per V pair and RHS
+                # (sd = td = 3), per V pair, frequency and RHS
                 w = f.work_counts()
-                flops = 8.0 * 2304 * 9 * w["vlist_pairs"] * NRHS
+                flops = 8.0 * spectrum_len(8, prec == "mixed") * 9 * w["vlist_pairs"] * NRHS
                 peak = PEAK_TF["fp32" if prec == "mixed" else "fp64"]
                 tf = flops / (ph["VListHadamard"] * 1e-3) / 1e12
                 res["%s_first_forest_hadamard" % prec] = {

@@ -25,6 +25,64 @@ constexpr int kNearChunk = 128;
 constexpr int kMaxNr = 8;
 __host__ __device__ constexpr int rhs_group(int nr) { return nr <= 1 ? 1 : nr <= 2 ? 2 : nr <= 4 ? 4 : 8; }
 constexpr int kPairStride = 224;  // 216 halo slots rounded up to a multiple of 8
+
+// Packed half spectrum of the mixed path.  The r2c layout of an n^3 real grid
+// (n = 2 np) holds (np + 1) n^2 values, frequency (ka, kb, kc), kc in [0, np].
+// In the planes kc = 0 and kc = np the value at (-ka, -kb) is the conjugate of
+// the one at (ka, kb), and so is every product of two such spectra, so the
+// V-list sum there is needed on one of each pair only: the packed form keeps
+// n^2 np + 4 values (868 of 1,008 at np = 6).
+//   kc in [1, np-1]: (ka n + kb)(np - 1) + kc - 1;
+//   kc in {0, np}:   base n^2 (np-1) (+ n^2/2 + 2 for kc = np), then ka = 0 with
+//                    kb in [0, np], ka in [1, np-1] with every kb, ka = np with
+//                    kb in [0, np].
+// spec_pack returns 2 * packed index + 1 when the value at (ka, kb, kc) is the
+// conjugate of the stored one (its mirror is the representative).
+__host__ __device__ constexpr int spec_packed_len(int np) { return 4 * np * np * np + 4; }
+__host__ __device__ constexpr int spec_pack(int np, int ka, int kb, int kc) {
+  const int n = 2 * np;
+  if (kc > 0 && kc < np) return ((ka * n + kb) * (np - 1) + kc - 1) << 1;
+  int a = ka, b = kb, cj = 0;
+  if (a > np || ((a == 0 || a == np) && b > np)) {
+    a = (n - a) % n;
+    b = (n - b) % n;
+    cj = 1;
+  }
+  const int base = n * n * (np - 1) + (kc ? n * n / 2 + 2 : 0);
+  const int p = a == 0 ? b : a == np ? np + 1 + (np - 1) * n + b : np + 1 + (a - 1) * n + b;
+  return ((base + p) << 1) | cj;
+}
+// spec_pack along one line e = kb (np + 1) + kc of the transforms, with the
+// line's part computed once: in the planes kc in [1, np-1] frequency ka sits at
+// lin + ka n (np - 1); in the planes kc = 0, np at pb + ka n for ka in [1, np-1],
+// mirrored at pm + (n - ka) n for ka in [np+1, n-1], and at z0, zh for ka = 0, np
+// (mirrored when kb > np).  With ka a constant each index is one add or select.
+template <int NP>
+struct SpecLine {
+  static constexpr int N = 2 * NP;
+  int inner, hi, lin, pb, pm, z0, zh;
+  __host__ __device__ explicit SpecLine(int e) {
+    const int kb = e / (NP + 1), kc = e % (NP + 1);
+    inner = kc > 0 && kc < NP;
+    hi = kb > NP;
+    lin = kb * (NP - 1) + kc - 1;
+    const int base = N * N * (NP - 1) + (kc ? N * N / 2 + 2 : 0);
+    const int bm = kb ? N - kb : 0;  // the mirror's kb
+    pb = base + NP + 1 - N + kb;
+    pm = base + NP + 1 - N + bm;
+    const int bz = hi ? bm : kb;
+    z0 = base + bz;
+    zh = base + NP + 1 + (NP - 1) * N + bz;
+  }
+  // a line of the planes kc = 0, np
+  __host__ __device__ int plane_index(int ka) const {
+    return ka == 0 ? z0 : ka == NP ? zh : ka < NP ? pb + ka * N : pm + (N - ka) * N;
+  }
+  __host__ __device__ bool plane_conj(int ka) const { return ka == 0 || ka == NP ? hi != 0 : ka > NP; }
+  __host__ __device__ int index(int ka) const { return inner ? lin + ka * N * (NP - 1) : plane_index(ka); }
+  // the value at ka is the conjugate of the stored one (not stored by this line)
+  __host__ __device__ bool conj(int ka) const { return !inner && plane_conj(ka); }
+};
 // Source item code: (node << 2) | kind
 enum ItemKind { ITEM_EXACT = 0, ITEM_UE = 1, ITEM_DE = 2 };
 

@@ -765,10 +765,11 @@ int FMM3d::upload_state() {
       if ((rc = check(espec_d_.alloc(sizeof(double2) * nt * op.H), "alloc V spectra"))) return rc;
       m2l32_d_.release(); dspec32_d_.release(); espec32_d_.release();
     } else {
-      std::vector<float> m2l32(op.m2l.begin(), op.m2l.end());
+      const std::vector<float> m2l32 = pack_spectra32(op.m2l, op.np);
       KIFMM_UP(m2l32_d_, m2l32, "upload M2L (FP32)");
-      if ((rc = check(dspec32_d_.alloc(sizeof(float2) * ns * op.H), "alloc V spectra"))) return rc;
-      if ((rc = check(espec32_d_.alloc(sizeof(float2) * nt * op.H), "alloc V spectra"))) return rc;
+      const size_t hc = size_t(spec_packed_len(op.np));
+      if ((rc = check(dspec32_d_.alloc(sizeof(float2) * ns * hc), "alloc V spectra"))) return rc;
+      if ((rc = check(espec32_d_.alloc(sizeof(float2) * nt * hc), "alloc V spectra"))) return rc;
       dspec_d_.release(); espec_d_.release();
     }
   }
@@ -803,7 +804,8 @@ int FMM3d::upload_vgroups(VGroups& vg) {
 // check values of the targets, added to).
 void FMM3d::vlist_args(VlistArgs& v, const VGroups& vg) const {
   const Operators& op = *ops_;
-  v.np = op.np; v.n = op.n; v.H = op.H; v.sd = op.sd; v.td = op.td;
+  v.np = op.np; v.n = op.n; v.sd = op.sd; v.td = op.td;
+  v.H = precision_ == PREC_MIXED ? spec_packed_len(op.np) : op.H;  // FP32 spectra are packed
   v.nsrc = int(vg.src_node.size()); v.ntrg = int(vg.trg_node.size()); v.ngrp = vg.ngrp();
   v.src_node = vg.src_node_d.as<int>(); v.trg_node = vg.trg_node_d.as<int>();
   v.depth = depth_d_.as<int>(); v.root_level = tree_.root_level;
@@ -847,6 +849,23 @@ double* FMM3d::splitk_for(const GemmArgs& g) {
   if (g.ksplit <= 1) return nullptr;
   const size_t need = size_t(g.ksplit) * size_t(g.ncol) * size_t(g.M) * sizeof(double);
   return need <= splitk_d_.bytes() ? splitk_d_.as<double>() : nullptr;
+}
+
+std::vector<float> pack_spectra32(const std::vector<double>& full, int np) {
+  const int n = 2 * np, hh = np + 1, h = n * n * hh, hc = spec_packed_len(np);
+  const size_t rows = full.size() / (2 * size_t(h));
+  std::vector<float> out(rows * size_t(hc) * 2);
+  for (size_t r = 0; r < rows; r++)
+    for (int ka = 0; ka < n; ka++)
+      for (int kb = 0; kb < n; kb++)
+        for (int kc = 0; kc < hh; kc++) {
+          const int code = spec_pack(np, ka, kb, kc);
+          if (code & 1) continue;  // the conjugate of a stored value
+          const size_t src = (r * h + size_t((ka * n + kb) * hh + kc)) * 2, dst = (r * hc + size_t(code >> 1)) * 2;
+          out[dst] = float(full[src]);
+          out[dst + 1] = float(full[src + 1]);
+        }
+  return out;
 }
 
 void split_k(GemmArgs& g) {
@@ -950,8 +969,9 @@ int FMM3d::reserve_rhs(int nrhs) {
     if ((rc = grow(dspec_d_, sizeof(double2) * nsv * op.H, "alloc V spectra"))) return rc;
     if ((rc = grow(espec_d_, sizeof(double2) * ntv * op.H, "alloc V spectra"))) return rc;
   } else {
-    if ((rc = grow(dspec32_d_, sizeof(float2) * nsv * op.H, "alloc V spectra"))) return rc;
-    if ((rc = grow(espec32_d_, sizeof(float2) * ntv * op.H, "alloc V spectra"))) return rc;
+    const size_t hc = size_t(spec_packed_len(op.np));
+    if ((rc = grow(dspec32_d_, sizeof(float2) * nsv * hc, "alloc V spectra"))) return rc;
+    if ((rc = grow(espec32_d_, sizeof(float2) * ntv * hc, "alloc V spectra"))) return rc;
   }
   rhs_cap_ = nrhs;
   return 0;

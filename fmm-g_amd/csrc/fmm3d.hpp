@@ -58,6 +58,11 @@ struct VGroups {
 // ones leave: x = p0 + p1 + p2 to about 2^-24 |x|.
 void split_bf16x3(double x, uint16_t* p);
 
+// FP32 copy of FP64 spectra rows (each row H complex values in the r2c layout
+// (ka n + kb) h + kc, re and im interleaved) in the packed half-spectrum
+// layout of the mixed path (spec_pack): rows x spec_packed_len(np) complex.
+std::vector<float> pack_spectra32(const std::vector<double>& full, int np);
+
 // Split K of a mixed-path level GEMM launch that would have too few workgroups.
 void split_k(GemmArgs& g);
 // The split of a launch over nrhs right-hand sides is the one-RHS launch's

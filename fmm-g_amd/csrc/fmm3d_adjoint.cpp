@@ -195,7 +195,7 @@ int FMM3d::build_adjoint() {
         }
     KIFMM_UP(m2lT_d_, t, "upload transposed M2L");
     if (precision_ == PREC_MIXED) {
-      std::vector<float> t32(t.begin(), t.end());
+      const std::vector<float> t32 = pack_spectra32(t, op.np);
       KIFMM_UP(m2lT32_d_, t32, "upload transposed M2L (FP32)");
     }
     std::vector<double> z(size_t(op.td), 0.0);
@@ -217,11 +217,12 @@ int FMM3d::build_adjoint() {
           (rc = check(espec_d_.alloc(sizeof(double2) * ntrg * op.H), "alloc V spectra")))
         return rc;
     } else {
-      if (dspec32_d_.bytes() < sizeof(float2) * nsrc * op.H &&
-          (rc = check(dspec32_d_.alloc(sizeof(float2) * nsrc * op.H), "alloc V spectra")))
+      const size_t hc = size_t(spec_packed_len(op.np));
+      if (dspec32_d_.bytes() < sizeof(float2) * nsrc * hc &&
+          (rc = check(dspec32_d_.alloc(sizeof(float2) * nsrc * hc), "alloc V spectra")))
         return rc;
-      if (espec32_d_.bytes() < sizeof(float2) * ntrg * op.H &&
-          (rc = check(espec32_d_.alloc(sizeof(float2) * ntrg * op.H), "alloc V spectra")))
+      if (espec32_d_.bytes() < sizeof(float2) * ntrg * hc &&
+          (rc = check(espec32_d_.alloc(sizeof(float2) * ntrg * hc), "alloc V spectra")))
         return rc;
     }
   }

@@ -397,6 +397,10 @@ __global__ __launch_bounds__(64) void k_fft_fwd(VlistArgs v, C2* dspec) {
   // mixed path at np = 6: the y and z line transforms by two 6-point DFTs
   // (the FP64 path keeps the ascending-order sums of the reference layout)
   constexpr bool kFast = NP == 6 && std::is_same<C2, float2>::value;
+  // mixed path: packed half spectrum (spec_pack), the FP64 path the full r2c
+  // layout (ka n + kb) h + kc
+  constexpr bool kPack = std::is_same<C2, float2>::value;
+  constexpr int HS = kPack ? spec_packed_len(NP) : N * N * HH;
   constexpr int NSAM = NP * NP * NP - (NP - 2) * (NP - 2) * (NP - 2);
   // mixed path: the line results between passes are kept in LDS as FP32 (the
   // spectra are stored in FP32 anyway), which halves the LDS of a workgroup
@@ -475,25 +479,51 @@ __global__ __launch_bounds__(64) void k_fft_fwd(VlistArgs v, C2* dspec) {
       }
     }
     __syncthreads();
-    // along z (a) -> global, frequency index (ka*n+kb)*h+kc; line (kb, kc)
-    C2* out = dspec + (int64_t(slot) * v.sd + s) * (N * N * HH);
-    for (int e = lane; e < N * HH; e += 64) {
+    // along z (a) -> global, frequency (ka, kb, kc); line (kb, kc)
+    C2* out = dspec + (int64_t(slot) * v.sd + s) * HS;
+    auto zline = [&](int e, double2 (&o)[N]) {  // the n outputs of line e = kb h + kc
       double2 in[NP];
 #pragma unroll
       for (int a = 0; a < NP; a++) in[a] = load_c(&t2[a * N * HH + e]);
       if constexpr (kFast) {
-        double2 o12[N];
-        pad12_fwd(in, w, o12);
-#pragma unroll
-        for (int ka = 0; ka < N; ka++) store_c(out + ka * N * HH + e, o12[ka]);
+        pad12_fwd(in, w, o);
       } else {
 #pragma unroll
         for (int ka = 0; ka < N; ka++) {
           double2 acc = make_double2(0, 0);
 #pragma unroll
           for (int a = 0; a < NP; a++) acc = cadd(acc, cmul(in[a], w[((a + O) * ka) % N]));
-          store_c(out + ka * N * HH + e, acc);
+          o[ka] = acc;
         }
+      }
+    };
+    if constexpr (kPack) {
+      // the lines of the planes kc in [1, np-1] first: line li = kb (np-1) +
+      // kc - 1 stores frequency ka at li + ka NI; then the lines p = 2 kb +
+      // (kc == np) of the planes kc = 0, np, which store ka in [0, np] without
+      // the mirrors.  Every lane round is of one kind.
+      constexpr int NI = N * (NP - 1);
+      for (int li = lane; li < NI; li += 64) {
+        double2 o[N];
+        zline((li / (NP - 1)) * HH + li % (NP - 1) + 1, o);
+#pragma unroll
+        for (int ka = 0; ka < N; ka++) store_c(out + li + ka * NI, o[ka]);
+      }
+      for (int p = lane; p < 2 * N; p += 64) {
+        const int e = (p >> 1) * HH + (p & 1) * NP;
+        const SpecLine<NP> L(e);
+        double2 o[N];
+        zline(e, o);
+#pragma unroll
+        for (int ka = 0; ka <= NP; ka++)
+          if (!L.plane_conj(ka)) store_c(out + L.plane_index(ka), o[ka]);
+      }
+    } else {
+      for (int e = lane; e < N * HH; e += 64) {
+        double2 o[N];
+        zline(e, o);
+#pragma unroll
+        for (int ka = 0; ka < N; ka++) store_c(out + ka * N * HH + e, o[ka]);
       }
     }
   }
@@ -561,6 +591,8 @@ template <int NP, typename C2>
 __global__ __launch_bounds__(64) void k_fft_inv(VlistArgs v, const C2* espec) {
   constexpr int N = 2 * NP, HH = NP + 1, O = NP / 2;
   constexpr bool kFast = NP == 6 && std::is_same<C2, float2>::value;
+  constexpr bool kPack = std::is_same<C2, float2>::value;
+  constexpr int HS = kPack ? spec_packed_len(NP) : N * N * HH;
   constexpr int NSAM = NP * NP * NP - (NP - 2) * (NP - 2) * (NP - 2);
   using LT = typename std::conditional<std::is_same<C2, float2>::value, float2, double2>::type;
   __shared__ LT y1[NP * N * HH];
@@ -580,17 +612,16 @@ __global__ __launch_bounds__(64) void k_fft_inv(VlistArgs v, const C2* espec) {
   const int lvl = v.depth[node] + v.root_level;
   for (int i = 0; i < v.td; i++) {
     const double oscale = v.deg_out ? ldexp(1.0, int(lvl * v.deg_out[i])) : 1.0;
-    const C2* in = espec + (int64_t(slot) * v.td + i) * (N * N * HH);
+    const C2* in = espec + (int64_t(slot) * v.td + i) * HS;
     __syncthreads();
-    // along z (a), only block rows a' in [0,np); line (kb, kc).  When the n*h
-    // lines fit two rounds of the 64 lanes (np <= 7), both rounds' inputs are
-    // loaded before either is transformed, so the load latency is paid once
-    // per node.
-    if constexpr (N * HH > 128) {
-      for (int e = lane; e < N * HH; e += 64) {
-        double2 x[N];
+    // along z (a), only block rows a' in [0,np); line (kb, kc)
+    auto zline = [&](const double2 (&x)[N], int e) {  // line e = kb h + kc from its n inputs
+      if constexpr (kFast) {
+        double2 o6[NP];
+        pad12_inv(x, w, o6);
 #pragma unroll
-        for (int ka = 0; ka < N; ka++) x[ka] = load_c(in + ka * N * HH + e);
+        for (int a = 0; a < NP; a++) store_c(&y1[a * N * HH + e], o6[a]);
+      } else {
 #pragma unroll
         for (int a = 0; a < NP; a++) {
           double2 acc = make_double2(0, 0);
@@ -599,7 +630,70 @@ __global__ __launch_bounds__(64) void k_fft_inv(VlistArgs v, const C2* espec) {
           store_c(&y1[a * N * HH + e], acc);
         }
       }
+    };
+    if constexpr (kPack) {
+      // packed spectrum: the lines of the planes kc in [1, np-1] (li = kb (np-1)
+      // + kc - 1 reads frequency ka at li + ka NI), then those of the planes
+      // kc = 0, np (p = 2 kb + (kc == np), a value that is not stored is the
+      // conjugate of its mirror), so each lane round is of one kind.  With one
+      // round of each (np <= 6) both rounds' inputs are loaded before either is
+      // transformed.
+      constexpr int NI = N * (NP - 1);
+      auto eline = [](int li) { return (li / (NP - 1)) * HH + li % (NP - 1) + 1; };
+      auto eplane = [](int p) { return (p >> 1) * HH + (p & 1) * NP; };
+      auto ldp = [&](const SpecLine<NP>& L, int ka) -> C2 {
+        C2 t = in[L.plane_index(ka)];
+        if (L.plane_conj(ka)) t.y = -t.y;
+        return t;
+      };
+      if constexpr (NI <= 64 && 2 * N <= 64) {
+        const int li = lane < NI ? lane : 0, p = lane < 2 * N ? lane : 0;
+        const SpecLine<NP> L(eplane(p));
+        C2 ri[N], rp[N];
+#pragma unroll
+        for (int ka = 0; ka < N; ka++) ri[ka] = in[li + ka * NI];
+#pragma unroll
+        for (int ka = 0; ka < N; ka++) rp[ka] = ldp(L, ka);
+        double2 x[N];
+        if (lane < NI) {
+#pragma unroll
+          for (int ka = 0; ka < N; ka++) x[ka] = load_c(&ri[ka]);
+          zline(x, eline(li));
+        }
+        if (lane < 2 * N) {
+#pragma unroll
+          for (int ka = 0; ka < N; ka++) x[ka] = load_c(&rp[ka]);
+          zline(x, eplane(p));
+        }
+      } else {
+        for (int li = lane; li < NI; li += 64) {
+          double2 x[N];
+#pragma unroll
+          for (int ka = 0; ka < N; ka++) x[ka] = load_c(in + li + ka * NI);
+          zline(x, eline(li));
+        }
+        for (int p = lane; p < 2 * N; p += 64) {
+          const SpecLine<NP> L(eplane(p));
+          double2 x[N];
+#pragma unroll
+          for (int ka = 0; ka < N; ka++) {
+            const C2 t = ldp(L, ka);
+            x[ka] = load_c(&t);
+          }
+          zline(x, eplane(p));
+        }
+      }
+    } else if constexpr (N * HH > 128) {
+      for (int e = lane; e < N * HH; e += 64) {
+        double2 x[N];
+#pragma unroll
+        for (int ka = 0; ka < N; ka++) x[ka] = load_c(in + ka * N * HH + e);
+        zline(x, e);
+      }
     } else {
+      // the n h lines fit two rounds of the 64 lanes (np <= 7): both rounds'
+      // inputs are loaded before either is transformed, so the load latency is
+      // paid once per node
       const int e0 = lane, e1 = lane + 64;
       const bool has1 = e1 < N * HH;
       C2 r0[N], r1[N];
@@ -614,20 +708,7 @@ __global__ __launch_bounds__(64) void k_fft_inv(VlistArgs v, const C2* espec) {
         double2 x[N];
 #pragma unroll
         for (int ka = 0; ka < N; ka++) x[ka] = load_c(half ? &r1[ka] : &r0[ka]);
-        if constexpr (kFast) {
-          double2 o6[NP];
-          pad12_inv(x, w, o6);
-#pragma unroll
-          for (int a = 0; a < NP; a++) store_c(&y1[a * N * HH + e], o6[a]);
-        } else {
-#pragma unroll
-          for (int a = 0; a < NP; a++) {
-            double2 acc = make_double2(0, 0);
-#pragma unroll
-            for (int ka = 0; ka < N; ka++) acc = cadd(acc, cmul(x[ka], w[((a + O) * ka) % N]));
-            store_c(&y1[a * N * HH + e], acc);
-          }
-        }
+        zline(x, e);
       }
     }
     __syncthreads();
