@@ -143,6 +143,10 @@ struct GemmArgs {
   const double* B; const int* bidx; int64_t ldb; const double* kscale; int ksd;
   double* C; const int* cidx; int64_t ldc; const double* rscale; int rsd;
   int ncol; int accumulate;
+  // two-part B (mixed path, k1 > 0): rows [0, k1) of column j are
+  // B1[cidx[j] * ldb1 + k], unscaled; rows [k1, K) are B[bidx[j] * ldb + k - k1]
+  // scaled by kscale[(k - k1) % ksd].  k1 = 0: B alone, as above.
+  const double* B1; int64_t ldb1; int k1;
   // several matrices in one launch: matrix z is A + z*a_zstride and owns the
   // columns [zoff[z], zoff[z+1]); zoff == nullptr means one matrix, all columns
   const int* zoff; int nz; int64_t a_zstride; int max_zcol;

@@ -44,6 +44,18 @@ FMM3d::~FMM3d() {
   for (auto ev : events_) (void)hipEventDestroy(ev);
 }
 
+// Rows and depth of the forward passes' level GEMM steps, by matrix kind.
+static void step_shape(const Operators& op, int mat, int& M, int& K) {
+  switch (mat) {
+    case 0: M = op.dat(UC); K = 8 * op.dat(UE); break;             // M2M
+    case 1: M = op.dat(UE); K = op.dat(UC); break;                 // upward pseudo-inverse
+    case 8: M = op.dat(UE); K = 8 * op.dat(UE); break;             // P_up [M_0 ... M_7]
+    case 2: M = op.dat(DE); K = op.dat(DC); break;                 // downward pseudo-inverse
+    case 9: M = op.dat(DE); K = op.dat(DC) + op.dat(DE); break;    // [P_dn | P_dn L2L_z]
+    default: M = op.dat(DC); K = op.dat(DE);                       // L2L
+  }
+}
+
 void FMM3d::drop_graph() {
   if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
   if (graph_) (void)hipGraphDestroy(graph_);
@@ -437,7 +449,7 @@ int FMM3d::build_tasks() {
   const int rl = t.root_level;
   std::vector<std::vector<int>> by_depth(t.max_depth + 1);
   for (int i = 0; i < N; i++) by_depth[t.depth[i]].push_back(i);
-  const bool fuse_up = precision_ == PREC_MIXED && trans_terms_ == 0;
+  const bool fuse_up = precision_ == PREC_MIXED && trans_terms_ == 0, fuse_dn = fuse_up;
   for (int d = t.max_depth; d >= 1; d--) {  // :557-595, children before parents
     std::vector<int> mb, mc, pb, lb;
     for (int i : by_depth[d]) {
@@ -468,6 +480,28 @@ int FMM3d::build_tasks() {
         int p = t.parent[i], c = i - t.child[p];
         lb[c].push_back(p); lc[c].push_back(i);
       }
+    }
+    if (d >= 3 && fuse_dn) {
+      // mixed path: DE straight from DC and the parent's DE through the 8
+      // matrices [P_dn | P_dn L2L_z], one launch a level (no DC round trip)
+      const int ks = scale_vec(d - 1 + rl, +1);
+      std::vector<int> b, c;
+      int zoff[9] = {0}, mz = 0;
+      for (int q = 0; q < 8; q++) {
+        b.insert(b.end(), lb[q].begin(), lb[q].end());
+        c.insert(c.end(), lc[q].begin(), lc[q].end());
+        zoff[q + 1] = int(b.size());
+        mz = std::max(mz, int(lb[q].size()));
+      }
+      work_.l2l_cols += double(c.size());
+      work_.dn_cols += double(pb.size());
+      if (!b.empty()) {
+        add_step(dn_steps_, 9, 2, 2, b, c, ks, scale_vec(d + rl, -1), 0);
+        dn_steps_.back().zoff_off = int64_t(lvl_idx_h_.size());
+        dn_steps_.back().max_zcol = mz;
+        lvl_idx_h_.insert(lvl_idx_h_.end(), zoff, zoff + 9);
+      }
+      continue;
     }
     if (d >= 3) {  // the 8 child matrices of the level in one launch
       int ks = scale_vec(d - 1 + rl, +1);
@@ -705,8 +739,25 @@ int FMM3d::upload_state() {
         for (int m = 0; m < ue; m++) w[size_t(k) * ue + m] += op.pinv_up[size_t(j) * ue + m] * mjk;
       }
     KIFMM_UP(wup_d_, w, "upload P_up M2M");
+    // W_z = [P_dn | P_dn L2L_z] (nDE x (nDC + nDE), column-major), z = 0..7
+    const int de = op.dat(DE), dc = op.dat(DC), kw = dc + de;
+    std::vector<double> wd(size_t(8) * de * kw, 0.0);
+#pragma omp parallel for
+    for (int zk = 0; zk < 8 * kw; zk++) {
+      const int z = zk / kw, k = zk % kw;
+      double* col = wd.data() + (size_t(z) * kw + k) * de;
+      if (k < dc) {
+        for (int m = 0; m < de; m++) col[m] = op.pinv_dn[size_t(k) * de + m];
+      } else {
+        const double* l = op.l2l.data() + size_t(z) * dc * de + size_t(k - dc) * dc;
+        for (int j = 0; j < dc; j++)
+          for (int m = 0; m < de; m++) col[m] += op.pinv_dn[size_t(j) * de + m] * l[j];
+      }
+    }
+    KIFMM_UP(wdn_d_, wd, "upload P_dn L2L");
   } else {
     wup_d_.release();
+    wdn_d_.release();
   }
   KIFMM_UP(l2l_d_, op.l2l, "upload L2L");
   if (precision_ == PREC_MIXED && trans_terms_ > 0) {
@@ -741,13 +792,7 @@ int FMM3d::upload_state() {
     for (const auto* steps : {&up_steps_, &dn_steps_})
       for (const GemmStep& st : *steps) {
         GemmArgs g{};
-        switch (st.mat) {
-          case 0: g.M = op.dat(UC); g.K = 8 * op.dat(UE); break;
-          case 1: g.M = op.dat(UE); g.K = op.dat(UC); break;
-          case 8: g.M = op.dat(UE); g.K = 8 * op.dat(UE); break;
-          case 2: g.M = op.dat(DE); g.K = op.dat(DC); break;
-          default: g.M = op.dat(DC); g.K = op.dat(DE);
-        }
+        step_shape(op, st.mat, g.M, g.K);
         g.ncol = st.ncol; g.max_zcol = st.ncol;
         if (st.zoff_off >= 0) { g.zoff = &st.ncol; g.nz = 8; g.max_zcol = st.max_zcol; }  // only tested for null
         split_k(g);
@@ -1024,13 +1069,7 @@ int FMM3d::prepare_rhs_steps(int nrhs) {
     for (const auto* steps : {&up_steps_r_, &dn_steps_r_})
       for (const GemmStep& st : *steps) {
         GemmArgs g{};
-        switch (st.mat) {
-          case 0: g.M = op.dat(UC); g.K = 8 * op.dat(UE); break;
-          case 1: g.M = op.dat(UE); g.K = op.dat(UC); break;
-          case 8: g.M = op.dat(UE); g.K = 8 * op.dat(UE); break;
-          case 2: g.M = op.dat(DE); g.K = op.dat(DC); break;
-          default: g.M = op.dat(DC); g.K = op.dat(DE);
-        }
+        step_shape(op, st.mat, g.M, g.K);
         g.ncol = st.ncol; g.max_zcol = st.ncol;
         if (st.zoff_off >= 0) { g.zoff = &st.ncol; g.nz = 8; g.max_zcol = st.max_zcol; }
         split_k_rhs(g, nrhs);
@@ -1119,20 +1158,25 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
   auto run_steps = [&](const std::vector<GemmStep>& steps) -> int {
     for (const GemmStep& st : steps) {
       GemmArgs g{};
+      step_shape(op, st.mat, g.M, g.K);
       switch (st.mat) {
-        case 0: g.M = op.dat(UC); g.K = 8 * op.dat(UE); g.A = m2m_d_.as<double>(); break;
-        case 1: g.M = op.dat(UE); g.K = op.dat(UC); g.A = pup_d_.as<double>(); break;
-        case 8: g.M = op.dat(UE); g.K = 8 * op.dat(UE); g.A = wup_d_.as<double>(); break;
-        case 2: g.M = op.dat(DE); g.K = op.dat(DC); g.A = pdn_d_.as<double>(); break;
-        default:
-          g.M = op.dat(DC); g.K = op.dat(DE);
-          g.A = l2l_d_.as<double>();
+        case 0: g.A = m2m_d_.as<double>(); break;
+        case 1: g.A = pup_d_.as<double>(); break;
+        case 8: g.A = wup_d_.as<double>(); break;
+        case 2: g.A = pdn_d_.as<double>(); break;
+        case 9:
+          // DE = [P_dn | P_dn L2L_z] [DC; DE of the parent]: the first K part
+          // is the output node's DC row
+          g.A = wdn_d_.as<double>();
+          g.B1 = bufs[3]; g.ldb1 = szs[3]; g.k1 = op.dat(DC);
+          break;
+        default: g.A = l2l_d_.as<double>();
       }
       g.max_zcol = st.ncol;
       if (st.zoff_off >= 0) {
         g.zoff = lvl_idx + st.zoff_off;
         g.nz = 8;
-        g.a_zstride = int64_t(op.dat(DC)) * op.dat(DE);
+        g.a_zstride = int64_t(g.M) * g.K;
         g.max_zcol = st.max_zcol;
       }
       g.B = bufs[st.src]; g.ldb = szs[st.src];

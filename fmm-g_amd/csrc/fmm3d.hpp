@@ -298,6 +298,7 @@ class FMM3d {
   DeviceBuffer m2mT_d_, pupT_d_, pdnT_d_, l2lT_d_;
   DeviceBuffer m2lT_d_, m2lT32_d_, deg0_d_;  // adjoint V list of the Stokes equivalent kernel
   DeviceBuffer wup_d_;  // mixed path: P_up [M_0 ... M_7]
+  DeviceBuffer wdn_d_;  // mixed path: [P_dn | P_dn L2L_z], z = 0..7
   DeviceBuffer direct_work_d_;  // direct sum: partial sums of the source chunks
   DeviceBuffer m2l32_d_, dspec32_d_, espec32_d_;  // mixed path: FP32 spectra
   std::vector<hipEvent_t> events_;

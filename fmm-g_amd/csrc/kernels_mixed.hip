@@ -1182,7 +1182,9 @@ hipError_t launch_hadamard_mixed(const VlistArgs& v, const float2* dspec, const 
 // With ksplit > 1, blockIdx.z = matrix * ksplit + K chunk: each chunk writes
 // its partial product to g.work, and k_splitk_reduce sums the chunks in chunk
 // order into C (deterministic), so a level with few columns still spreads over
-// enough workgroups.  A split launch without a workspace is rejected.  Operand
+// enough workgroups.  A split launch without a workspace is rejected.  With a
+// two-part B (k1 > 0: the downward step [P_dn | P_dn L2L_z] [DC; DE_parent])
+// the first k1 rows of a column come from B1 at the output node's row.  Operand
 // layout of the 16x16x4 instruction: lane l supplies A[l%16][l/16] and
 // B[l/16][l%16], and its result register r holds D[4r + l/16][l%16] (rows
 // interleaved across the four lane groups).
@@ -1214,9 +1216,12 @@ __global__ __launch_bounds__(GM_T, 2) void k_gemm_mfma(GemmArgs g) {
   // and the bounds mask and the K scale are applied when the slab is stored,
   // so no wait for them sits between the loads and the products
   double ra[GM_RA], rb[GM_RB], rs[GM_RB];
-  // the B column this thread stages (8 threads a column)
+  // the B column this thread stages (8 threads a column); with a two-part B its
+  // first k1 rows come from B1 at the output column's row
   const int jb = n0 + (tid >> 3);
-  const int64_t bc = jb < cend ? int64_t(g.bidx[jb]) * g.ldb : int64_t(-1);
+  const bool colok = jb < cend;
+  const double* bcol = colok ? g.B + int64_t(g.bidx[jb]) * g.ldb - g.k1 : g.B;
+  const double* bcol1 = colok && g.k1 > 0 ? g.B1 + int64_t(g.cidx[jb]) * g.ldb1 : g.B;
   auto fetch = [&](int k0) {
 #pragma unroll
     for (int u = 0; u < GM_RA; u++) {
@@ -1228,8 +1233,9 @@ __global__ __launch_bounds__(GM_T, 2) void k_gemm_mfma(GemmArgs g) {
 #pragma unroll
     for (int u = 0; u < GM_RB; u++) {
       const int kq = kq0 + u;
-      rb[u] = g.B[bc >= 0 && kq < kend ? bc + kq : 0];
-      rs[u] = g.kscale[kq % g.ksd];
+      const bool p1 = kq < g.k1;
+      rb[u] = colok && kq < kend ? (p1 ? bcol1 : bcol)[kq] : g.B[0];
+      rs[u] = p1 ? 1.0 : g.kscale[p1 ? 0 : (kq - g.k1) % g.ksd];
     }
   };
   auto store = [&](int buf, int k0) {
@@ -1242,7 +1248,7 @@ __global__ __launch_bounds__(GM_T, 2) void k_gemm_mfma(GemmArgs g) {
 #pragma unroll
     for (int u = 0; u < GM_RB; u++) {
       const int kq = k0 + GM_RB * (tid & 7) + u;
-      Bs[buf][GM_RB * (tid & 7) + u][tid >> 3] = bc >= 0 && kq < kend ? rb[u] * rs[u] : 0.0;
+      Bs[buf][GM_RB * (tid & 7) + u][tid >> 3] = colok && kq < kend ? rb[u] * rs[u] : 0.0;
     }
   };
   fetch(kbeg);
