@@ -37,8 +37,11 @@ constexpr int kPairStride = 224;  // 216 halo slots rounded up to a multiple of 
 //                    kb in [0, np], ka in [1, np-1] with every kb, ka = np with
 //                    kb in [0, np].
 // spec_pack returns 2 * packed index + 1 when the value at (ka, kb, kc) is the
-// conjugate of the stored one (its mirror is the representative).
-__host__ __device__ constexpr int spec_packed_len(int np) { return 4 * np * np * np + 4; }
+// conjugate of the stored one (its mirror is the representative).  A row is
+// padded with zeros to a multiple of 16 values (880 at np = 6), so that every
+// 64-frequency chunk of a row starts on a 128-byte line.
+__host__ __device__ constexpr int spec_packed_count(int np) { return 4 * np * np * np + 4; }
+__host__ __device__ constexpr int spec_packed_len(int np) { return (spec_packed_count(np) + 15) / 16 * 16; }
 __host__ __device__ constexpr int spec_pack(int np, int ka, int kb, int kc) {
   const int n = 2 * np;
   if (kc > 0 && kc < np) return ((ka * n + kb) * (np - 1) + kc - 1) << 1;

@@ -897,7 +897,7 @@ double* FMM3d::splitk_for(const GemmArgs& g) {
 }
 
 std::vector<float> pack_spectra32(const std::vector<double>& full, int np) {
-  const int n = 2 * np, hh = np + 1, h = n * n * hh, hc = spec_packed_len(np);
+  const int n = 2 * np, hh = np + 1, h = n * n * hh, hc = spec_packed_len(np);  // padding stays zero
   const size_t rows = full.size() / (2 * size_t(h));
   std::vector<float> out(rows * size_t(hc) * 2);
   for (size_t r = 0; r < rows; r++)

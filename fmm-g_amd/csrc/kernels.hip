@@ -518,6 +518,9 @@ __global__ __launch_bounds__(64) void k_fft_fwd(VlistArgs v, C2* dspec) {
         for (int ka = 0; ka <= NP; ka++)
           if (!L.plane_conj(ka)) store_c(out + L.plane_index(ka), o[ka]);
       }
+      // the row's padding (the Hadamard sweeps it with the other frequencies)
+      constexpr int NC = spec_packed_count(NP);
+      for (int q = NC + lane; q < HS; q += 64) store_c(out + q, make_double2(0, 0));
     } else {
       for (int e = lane; e < N * HH; e += 64) {
         double2 o[N];
