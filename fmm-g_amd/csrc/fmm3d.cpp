@@ -915,18 +915,19 @@ std::vector<float> pack_spectra32(const std::vector<double>& full, int np) {
 
 void split_k(GemmArgs& g) {
   // A launch's time is its tile count over the CUs times the serial K loop
-  // (~3 us per 16-deep slab).  Splitting K pays where that loop is long and the
-  // tiles few: the top levels (<= 16 tiles, any K) and the fused or plain M2M
-  // with K = 8 nUE >= 1024 up to two tiles per CU (<= 512 tiles; the two
-  // deepest parent levels of the 8M-point workload have ~165 tiles and ran one
-  // wave per SIMD unsplit).  Elsewhere the FP64 atomic adds of the partial
-  // products cost more than the split saves (measured).
+  // (~2-3 us per 16-deep slab when the launch is small: the slab's loads are
+  // not hidden).  Splitting K pays where that loop is long and the tiles few:
+  // the top levels (<= 16 tiles, any K: up to 64 chunks, down to one slab
+  // each) and the fused or plain M2M with K = 8 nUE >= 1024 up to two tiles per
+  // CU (<= 512 tiles; the two deepest parent levels of the 8M-point workload
+  // have ~165 tiles and ran one wave per SIMD unsplit).  Elsewhere the
+  // workspace round trip and the summing launch cost more than the split saves.
   const int64_t base = int64_t((g.M + 159) / 160) * ((g.max_zcol + 63) / 64) * (g.zoff ? g.nz : 1);
-  int64_t want = 0, cap = 16;
+  int64_t want = 0, cap = 64;
   if (base <= 16) want = 128;
   else if (g.K >= 1024 && base <= 512) { want = 512; cap = 8; }
   if (want == 0 || g.K <= 16) return;
-  int ks = int(std::min<int64_t>({cap, (want + base - 1) / base, g.K / 16}));
+  int ks = int(std::min<int64_t>({cap, (want + base - 1) / base, (g.K + 15) / 16}));
   if (ks <= 1) return;
   int chunk = ((g.K + ks - 1) / ks + 15) / 16 * 16;
   g.kchunk = chunk;

@@ -1294,8 +1294,17 @@ __global__ void k_splitk_reduce(const double* __restrict__ work, int ks, int nco
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= int64_t(ncol) * M) return;
   const int n = int(i / M), m = int(i % M);
+  // chunk order; the loads of eight chunks are issued before their adds
   double v = 0.0;
-  for (int z = 0; z < ks; z++) v += work[(int64_t(z) * ncol + n) * M + m];
+  int z = 0;
+  for (; z + 8 <= ks; z += 8) {
+    double t[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) t[u] = work[(int64_t(z + u) * ncol + n) * M + m];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v += t[u];
+  }
+  for (; z < ks; z++) v += work[(int64_t(z) * ncol + n) * M + m];
   double* dst = C + int64_t(cidx[n]) * ldc + m;
   *dst = accumulate ? *dst + v : v;
 }
