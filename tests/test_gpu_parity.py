@@ -181,6 +181,34 @@ def test_mixed_precision(kt, np_, coefs, gen, tol):
     assert out["mixed_err"] <= out["fp64_err"] + 2 * d
 
 
+@pytest.mark.parametrize("kt,np_,coefs,tol", [(111, 3, [1.0, 0.25], 1e-6), (111, 5, [1.0, 0.25], 1e-6),
+                                              (111, 7, [1.0, 0.25], 1e-6), (111, 8, [1.0, 0.25], 1e-6),
+                                              (511, 8, [1.0, 0.3], 2e-6)])
+def test_mixed_every_np(kt, np_, coefs, tol):
+    """The mixed path's packed half spectrum at the expansion orders the other
+    tests leave out: the transforms handle the interior-plane and the
+    mirrored-plane lines in lane rounds of their own (both rounds loaded
+    together up to np 6, one round at a time above), and the Hadamard sweeps
+    rows padded to a multiple of 16 frequencies.  Mixed against FP64 on the same
+    tree, and no less accurate against the direct sum beyond that difference."""
+    n = 8000
+    pos = W.gaussians(n, 95)
+    den = W.densities(n, O.src_dof(kt), 96, kind="normal")
+    print("case kt=%d np=%d mixed vs fp64" % (kt, np_), flush=True)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        f = kifmm.FMM3d(kt, coefs, np_, 32, 30, precision=prec)
+        f.set_points(pos)
+        f.setup()
+        out[prec] = f.evaluate(den)
+        out[prec + "_err"] = f.check(den, out[prec], 1000, seed=11)
+    d = rel(out["mixed"], out["fp64"])
+    print("  mixed vs fp64 %.3e, errors vs direct: fp64 %.3e mixed %.3e" % (d, out["fp64_err"], out["mixed_err"]),
+          flush=True)
+    assert d < tol
+    assert out["mixed_err"] <= out["fp64_err"] + 2 * d
+
+
 def test_mixed_near_cut_across_leaves():
     """The mixed near field drops the 1e-8 pair test where the host proves no
     pair can need it (only the target leaf's own sources can).  Points 5e-9
