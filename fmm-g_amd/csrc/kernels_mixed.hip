@@ -676,9 +676,15 @@ __global__ __launch_bounds__(kSurfMx) void k_surface_mx(SurfArgs a, KParams<floa
 }
 
 // ---------------------------------------------------------------------------
+typedef float tf2 __attribute__((ext_vector_type(2)));
+
 // V-list Hadamard with FP32 spectra: one wave per sibling group and 64
-// frequencies, lane = frequency, the 8 targets' accumulators per lane.
-template <int SD, int TD>
+// frequencies, lane = frequency, the 8 targets' accumulators per lane.  Spectra
+// rows are read through buffer resources, the (uniform) row offset in the
+// scalar offset and the lane's frequency in the vector offset, so the 16 loads
+// of a pair need no vector address arithmetic (the translation table always
+// fits 32-bit offsets; the source spectra when BUFX).
+template <int SD, int TD, bool BUFX>
 __global__ __launch_bounds__(64) void k_hadamard_mx(VlistArgs v, const float2* __restrict__ dspec,
                                                    const float2* __restrict__ m2l, float2* __restrict__ espec,
                                                    float nrm) {
@@ -688,6 +694,23 @@ __global__ __launch_bounds__(64) void k_hadamard_mx(VlistArgs v, const float2* _
   const int H = v.H;
   const bool ok = f < H;
   const int ff = ok ? f : 0;
+  const unsigned H8 = unsigned(H) * 8u, ff8 = unsigned(ff) * 8u;
+  const __amdgpu_buffer_rsrc_t rs_t =
+      __builtin_amdgcn_make_buffer_rsrc((void*)m2l, (short)0, int(343u * SD * TD * H8), 0x00020000);
+  const __amdgpu_buffer_rsrc_t rs_x = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)dspec, (short)0, int(unsigned(v.nsrc) * unsigned(v.nrhs) * SD * H8), 0x00020000);
+  auto ldt = [&](int row) -> float2 {
+    const tf2 t = __builtin_bit_cast(tf2, __builtin_amdgcn_raw_buffer_load_b64(rs_t, ff8, unsigned(row) * H8, 0));
+    return make_float2(t.x, t.y);
+  };
+  auto ldx = [&](int row) -> float2 {
+    if constexpr (BUFX) {
+      const tf2 x = __builtin_bit_cast(tf2, __builtin_amdgcn_raw_buffer_load_b64(rs_x, ff8, unsigned(row) * H8, 0));
+      return make_float2(x.x, x.y);
+    } else {
+      return dspec[int64_t(row) * H + ff];
+    }
+  };
   float2 acc[8][TD];
 #pragma unroll
   for (int t = 0; t < 8; t++)
@@ -701,18 +724,17 @@ __global__ __launch_bounds__(64) void k_hadamard_mx(VlistArgs v, const float2* _
     if (s < 0) continue;
     float2 d[SD];
 #pragma unroll
-    for (int j = 0; j < SD; j++) d[j] = dspec[(int64_t(s + soff) * SD + j) * H + ff];
+    for (int j = 0; j < SD; j++) d[j] = ldx((s + soff) * SD + j);
     const int rs = (word >> 8) & 0x3ff, tm = word >> 20;
 #pragma unroll
     for (int tau = 0; tau < 8; tau++) {
       if (!((tm >> tau) & 1)) continue;
       const int slot = rs - ((tau >> 2) * 49 + ((tau >> 1) & 1) * 7 + (tau & 1));
-      const float2* T = m2l + int64_t(slot) * SD * TD * H;
 #pragma unroll
       for (int i = 0; i < TD; i++)
 #pragma unroll
         for (int j = 0; j < SD; j++) {
-          const float2 t = T[int64_t(j * TD + i) * H + ff];
+          const float2 t = ldt(slot * (SD * TD) + j * TD + i);
           acc[tau][i].x = fmaf(t.x, d[j].x, fmaf(-t.y, d[j].y, acc[tau][i].x));
           acc[tau][i].y = fmaf(t.x, d[j].y, fmaf(t.y, d[j].x, acc[tau][i].y));
         }
@@ -761,7 +783,6 @@ __global__ __launch_bounds__(64) void k_hadamard_mx(VlistArgs v, const float2* _
 // loads need no vector address arithmetic (both tables must fit in 32-bit byte
 // offsets).
 constexpr int kTileG = 5;
-typedef float tf2 __attribute__((ext_vector_type(2)));
 
 template <bool BUF>
 __global__ __launch_bounds__(64) void k_m2l_tile(VlistArgs v, const float2* __restrict__ dspec,
@@ -1140,10 +1161,19 @@ hipError_t launch_hadamard_mixed(const VlistArgs& v, const float2* dspec, const 
       k_m2l_tile<false><<<grid2, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm, ntile, per_xcd);
   }
   else if (v.had_form == 0) {  // one wave per group, translation spectra read per pair
-    if (v.sd == 4 && v.td == 3) k_hadamard_mx<4, 3><<<grid, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm);
-    else if (v.sd == 3 && v.td == 3) k_hadamard_mx<3, 3><<<grid, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm);
-    else if (v.sd == 3 && v.td == 4) k_hadamard_mx<3, 4><<<grid, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm);
-    else return hipErrorInvalidValue;
+    // source spectra through a buffer resource when they fit 32-bit offsets
+    const bool bufx = uint64_t(v.nsrc) * v.nrhs * v.sd * v.H * 8 <= 0xFFFFFF00ull;
+#define KIFMM_PAIR(SD_, TD_)                                                                          \
+  if (v.sd == SD_ && v.td == TD_) {                                                                   \
+    if (bufx) k_hadamard_mx<SD_, TD_, true><<<grid, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm);     \
+    else k_hadamard_mx<SD_, TD_, false><<<grid, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm);         \
+    return hipGetLastError();                                                                          \
+  }
+    KIFMM_PAIR(4, 3)
+    KIFMM_PAIR(3, 3)
+    KIFMM_PAIR(3, 4)
+#undef KIFMM_PAIR
+    return hipErrorInvalidValue;
   } else {  // halo sweep over tiles of 1 or 2 groups
     const int G = v.had_form == 1 ? 1 : 2;
     const int ntile = int((nvg + G - 1) / G);
