@@ -191,6 +191,10 @@ struct VlistArgs {
   // reading the translation spectra per pair, 1 and 2 the halo sweep over tiles
   // of 1 or 2 groups
   int had_form = 0;
+  // the translation spectra of DOFs (j, i) and (i, j), i, j < 3, are equal (the
+  // 3 x 3 block G delta_ij + H x_i x_j of the Stokes and Navier equivalent
+  // kernels): the mixed per-pair Hadamard loads one of each pair
+  int sym3 = 0;
 };
 
 // Adjoint near field (source-centric): chunk = <= kNearChunk sources of a

@@ -812,6 +812,7 @@ int FMM3d::upload_state() {
     } else {
       const std::vector<float> m2l32 = pack_spectra32(op.m2l, op.np);
       KIFMM_UP(m2l32_d_, m2l32, "upload M2L (FP32)");
+      m2l_sym3_ = table_sym3(op.m2l, op.sd, op.td, op.H);
       const size_t hc = size_t(spec_packed_len(op.np));
       if ((rc = check(dspec32_d_.alloc(sizeof(float2) * ns * hc), "alloc V spectra"))) return rc;
       if ((rc = check(espec32_d_.alloc(sizeof(float2) * nt * hc), "alloc V spectra"))) return rc;
@@ -860,6 +861,7 @@ void FMM3d::vlist_args(VlistArgs& v, const VGroups& vg) const {
   v.mask = vg.mask_d.as<uint64_t>(); v.tgt = vg.tgt_d.as<int>();
   v.m2l = m2l_d_.as<double2>(); v.dspec = dspec_d_.as<double2>(); v.espec = espec_d_.as<double2>();
   v.had_form = tune_.vec_hadamard;
+  v.sym3 = m2l_sym3_ ? 1 : 0;
 }
 
 // timed: record the Hadamard's start and end events (forward pass with timing on).
@@ -894,6 +896,21 @@ double* FMM3d::splitk_for(const GemmArgs& g) {
   if (g.ksplit <= 1) return nullptr;
   const size_t need = size_t(g.ksplit) * size_t(g.ncol) * size_t(g.M) * sizeof(double);
   return need <= splitk_d_.bytes() ? splitk_d_.as<double>() : nullptr;
+}
+
+bool table_sym3(const std::vector<double>& t, int sd, int td, int H) {
+  if (sd < 3 || td < 3) return false;
+  const size_t per = size_t(sd) * td * H * 2, nslot = t.size() / per;
+  double big = 0.0, diff = 0.0;
+  for (double x : t) big = std::max(big, std::fabs(x));
+  for (size_t sl = 0; sl < nslot; sl++)
+    for (int j = 0; j < 3; j++)
+      for (int i = 0; i < j; i++) {
+        const double* a = t.data() + sl * per + size_t(j * td + i) * H * 2;
+        const double* b = t.data() + sl * per + size_t(i * td + j) * H * 2;
+        for (int f = 0; f < 2 * H; f++) diff = std::max(diff, std::fabs(a[f] - b[f]));
+      }
+  return diff <= 1e-12 * big;
 }
 
 std::vector<float> pack_spectra32(const std::vector<double>& full, int np) {

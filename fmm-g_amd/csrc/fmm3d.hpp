@@ -63,6 +63,11 @@ void split_bf16x3(double x, uint16_t* p);
 // layout of the mixed path (spec_pack): rows x spec_packed_len(np) complex.
 std::vector<float> pack_spectra32(const std::vector<double>& full, int np);
 
+// Whether the 3 x 3 DOF block of FP64 translation spectra rows ([slot][sd*td][H]
+// complex, DOF pair (j, i) at j * td + i) is symmetric, T(j, i) = T(i, j), to
+// 1e-12 of the table's largest value.
+bool table_sym3(const std::vector<double>& t, int sd, int td, int H);
+
 // Split K of a mixed-path level GEMM launch that would have too few workgroups.
 void split_k(GemmArgs& g);
 // The split of a launch over nrhs right-hand sides is the one-RHS launch's
@@ -301,6 +306,7 @@ class FMM3d {
   DeviceBuffer wdn_d_;  // mixed path: [P_dn | P_dn L2L_z], z = 0..7
   DeviceBuffer direct_work_d_;  // direct sum: partial sums of the source chunks
   DeviceBuffer m2l32_d_, dspec32_d_, espec32_d_;  // mixed path: FP32 spectra
+  bool m2l_sym3_ = false, m2lT_sym3_ = false;     // table_sym3 of the forward / transposed table
   std::vector<hipEvent_t> events_;
   bool use_graph_ = false;
   hipGraph_t graph_ = nullptr;

@@ -197,6 +197,7 @@ int FMM3d::build_adjoint() {
     if (precision_ == PREC_MIXED) {
       const std::vector<float> t32 = pack_spectra32(t, op.np);
       KIFMM_UP(m2lT32_d_, t32, "upload transposed M2L (FP32)");
+      m2lT_sym3_ = table_sym3(t, td3, sd4, H);  // transposed: td DOFs in, sd out
     }
     std::vector<double> z(size_t(op.td), 0.0);
     KIFMM_UP(deg0_d_, z, "upload degrees");
@@ -330,6 +331,7 @@ int FMM3d::evaluateAdjoint(const double* d_trg_in, double* d_src_out, hipStream_
     v.td = op.sd;
     v.m2l = m2lT_d_.as<double2>();
     m2l32 = m2lT32_d_.as<float2>();
+    v.sym3 = m2lT_sym3_ ? 1 : 0;
     v.deg = deg0_d_.as<double>();
     v.deg_out = deg_d_.as<double>();
   }

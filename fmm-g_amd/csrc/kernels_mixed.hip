@@ -683,8 +683,10 @@ typedef float tf2 __attribute__((ext_vector_type(2)));
 // rows are read through buffer resources, the (uniform) row offset in the
 // scalar offset and the lane's frequency in the vector offset, so the 16 loads
 // of a pair need no vector address arithmetic (the translation table always
-// fits 32-bit offsets; the source spectra when BUFX).
-template <int SD, int TD, bool BUFX>
+// fits 32-bit offsets; the source spectra when BUFX).  SYM: the table's DOF
+// pairs (j, i) and (i, j), i, j < 3, hold the same spectra, so 9 of 12 (Stokes)
+// or 6 of 9 (Navier) translation loads a pair serve all the products.
+template <int SD, int TD, bool BUFX, bool SYM>
 __global__ __launch_bounds__(64) void k_hadamard_mx(VlistArgs v, const float2* __restrict__ dspec,
                                                    const float2* __restrict__ m2l, float2* __restrict__ espec,
                                                    float nrm) {
@@ -730,11 +732,23 @@ __global__ __launch_bounds__(64) void k_hadamard_mx(VlistArgs v, const float2* _
     for (int tau = 0; tau < 8; tau++) {
       if (!((tm >> tau) & 1)) continue;
       const int slot = rs - ((tau >> 2) * 49 + ((tau >> 1) & 1) * 7 + (tau & 1));
+      float2 tv[SD][TD];
+#pragma unroll
+      for (int j = 0; j < SD; j++)
+#pragma unroll
+        for (int i = 0; i < TD; i++)
+          if (!(SYM && i < j && j < 3)) tv[j][i] = ldt(slot * (SD * TD) + j * TD + i);
+      if constexpr (SYM) {
+#pragma unroll
+        for (int j = 1; j < 3; j++)
+#pragma unroll
+          for (int i = 0; i < j; i++) tv[j][i] = tv[i][j];
+      }
 #pragma unroll
       for (int i = 0; i < TD; i++)
 #pragma unroll
         for (int j = 0; j < SD; j++) {
-          const float2 t = ldt(slot * (SD * TD) + j * TD + i);
+          const float2 t = tv[j][i];
           acc[tau][i].x = fmaf(t.x, d[j].x, fmaf(-t.y, d[j].y, acc[tau][i].x));
           acc[tau][i].y = fmaf(t.x, d[j].y, fmaf(t.y, d[j].x, acc[tau][i].y));
         }
@@ -1163,16 +1177,20 @@ hipError_t launch_hadamard_mixed(const VlistArgs& v, const float2* dspec, const 
   else if (v.had_form == 0) {  // one wave per group, translation spectra read per pair
     // source spectra through a buffer resource when they fit 32-bit offsets
     const bool bufx = uint64_t(v.nsrc) * v.nrhs * v.sd * v.H * 8 <= 0xFFFFFF00ull;
-#define KIFMM_PAIR(SD_, TD_)                                                                          \
-  if (v.sd == SD_ && v.td == TD_) {                                                                   \
-    if (bufx) k_hadamard_mx<SD_, TD_, true><<<grid, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm);     \
-    else k_hadamard_mx<SD_, TD_, false><<<grid, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm);         \
-    return hipGetLastError();                                                                          \
+#define KIFMM_PAIR_B(SD_, TD_, SYM_)                                                                          \
+  if (bufx) k_hadamard_mx<SD_, TD_, true, SYM_><<<grid, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm);         \
+  else k_hadamard_mx<SD_, TD_, false, SYM_><<<grid, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm);
+#define KIFMM_PAIR(SD_, TD_)                                                                                  \
+  if (v.sd == SD_ && v.td == TD_) {                                                                           \
+    if (v.sym3) { KIFMM_PAIR_B(SD_, TD_, true) }                                                              \
+    else { KIFMM_PAIR_B(SD_, TD_, false) }                                                                    \
+    return hipGetLastError();                                                                                  \
   }
     KIFMM_PAIR(4, 3)
     KIFMM_PAIR(3, 3)
     KIFMM_PAIR(3, 4)
 #undef KIFMM_PAIR
+#undef KIFMM_PAIR_B
     return hipErrorInvalidValue;
   } else {  // halo sweep over tiles of 1 or 2 groups
     const int G = v.had_form == 1 ? 1 : 2;
