@@ -195,6 +195,10 @@ struct VlistArgs {
   // 3 x 3 block G delta_ij + H x_i x_j of the Stokes and Navier equivalent
   // kernels): the mixed per-pair Hadamard loads one of each pair
   int sym3 = 0;
+  // the FP32 translation spectra of opposite offsets are conjugates, T(-s) =
+  // conj(T(s)) (scalar even kernel, table without the kernel grid's wrap
+  // entries): the mixed halo sweep serves offsets q and -q from one load
+  int mirror = 0;
 };
 
 // Adjoint near field (source-centric): chunk = <= kNearChunk sources of a

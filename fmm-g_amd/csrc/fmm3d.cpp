@@ -810,7 +810,16 @@ int FMM3d::upload_state() {
       if ((rc = check(espec_d_.alloc(sizeof(double2) * nt * op.H), "alloc V spectra"))) return rc;
       m2l32_d_.release(); dspec32_d_.release(); espec32_d_.release();
     } else {
-      const std::vector<float> m2l32 = pack_spectra32(op.m2l, op.np);
+      // scalar kernels: the FP32 table without the kernel grid's wrap entries
+      // (equal in every product an evaluate forms), whose opposite offsets are
+      // conjugates when the kernel is even
+      m2l_mirror_ = false;
+      std::vector<double> nowrap;
+      if (op.sd == 1 && op.td == 1) {
+        nowrap = m2l_table_nowrap(op);
+        m2l_mirror_ = table_mirror(nowrap, 1, op.H);
+      }
+      const std::vector<float> m2l32 = pack_spectra32(m2l_mirror_ ? nowrap : op.m2l, op.np);
       KIFMM_UP(m2l32_d_, m2l32, "upload M2L (FP32)");
       m2l_sym3_ = table_sym3(op.m2l, op.sd, op.td, op.H);
       const size_t hc = size_t(spec_packed_len(op.np));
@@ -862,6 +871,7 @@ void FMM3d::vlist_args(VlistArgs& v, const VGroups& vg) const {
   v.m2l = m2l_d_.as<double2>(); v.dspec = dspec_d_.as<double2>(); v.espec = espec_d_.as<double2>();
   v.had_form = tune_.vec_hadamard;
   v.sym3 = m2l_sym3_ ? 1 : 0;
+  v.mirror = m2l_mirror_ ? 1 : 0;
 }
 
 // timed: record the Hadamard's start and end events (forward pass with timing on).

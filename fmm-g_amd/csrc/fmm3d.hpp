@@ -307,6 +307,7 @@ class FMM3d {
   DeviceBuffer direct_work_d_;  // direct sum: partial sums of the source chunks
   DeviceBuffer m2l32_d_, dspec32_d_, espec32_d_;  // mixed path: FP32 spectra
   bool m2l_sym3_ = false, m2lT_sym3_ = false;     // table_sym3 of the forward / transposed table
+  bool m2l_mirror_ = false;  // the FP32 table is the wrap-free one and table_mirror holds
   std::vector<hipEvent_t> events_;
   bool use_graph_ = false;
   hipGraph_t graph_ = nullptr;

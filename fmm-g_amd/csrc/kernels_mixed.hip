@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "device.hpp"
 #include "gemm.hpp"
@@ -798,7 +799,7 @@ __global__ __launch_bounds__(64) void k_hadamard_mx(VlistArgs v, const float2* _
 // offsets).
 constexpr int kTileG = 5;
 
-template <bool BUF>
+template <bool BUF, bool MIRROR>
 __global__ __launch_bounds__(64) void k_m2l_tile(VlistArgs v, const float2* __restrict__ dspec,
                                                  const float2* __restrict__ m2l, float2* __restrict__ espec,
                                                  float nrm, int ntile, int per_xcd) {
@@ -869,12 +870,11 @@ __global__ __launch_bounds__(64) void k_m2l_tile(VlistArgs v, const float2* __re
   };
   int sc[8];
   float2 xc[8];
-  fetch(0, 0, sc, xc);
-  for (int q = 0; q < 27; q++) {
-    if (q == 13) continue;  // the parent itself: siblings are adjacent
+  float ta[27], tb[27], ts[27];
+  // the 27 translation spectra 2q + [-1,1]^3 of colleague offset q
+  auto load_t = [&](int q) {
     const int qx = q / 9 - 1, qy = (q / 3) % 3 - 1, qz = q % 3 - 1;
     const int sbase = (2 * qx + 3) * 49 + (2 * qy + 3) * 7 + (2 * qz + 3);
-    float ta[27], tb[27], ts[27];
 #pragma unroll
     for (int d = 0; d < 27; d++) {
       const int sl = sbase + (d / 9 - 1) * 49 + ((d / 3) % 3 - 1) * 7 + (d % 3 - 1);
@@ -883,12 +883,20 @@ __global__ __launch_bounds__(64) void k_m2l_tile(VlistArgs v, const float2* __re
       tb[d] = t.y;
       ts[d] = t.x + t.y;
     }
+  };
+  // the G groups' products for colleague offset q; qnext: the offset whose
+  // first group's sources are loaded during the last group (-1: none).  With
+  // mirrored (MIRROR), q is the opposite of the offset whose translation
+  // spectra are in the registers: the spectrum of 2q + delta is the conjugate
+  // of the one of -(2q + delta), register 26 - d, and ts must hold ta - tb.
+  auto sweep = [&](auto mirrored, int q, int qnext) {
+    constexpr bool M = decltype(mirrored)::value;
 #pragma unroll
     for (int gi = 0; gi < G; gi++) {
       int sn[8];
       float2 xn[8];
-      const int qn = gi + 1 < G ? q : (q + 1 == 13 ? 14 : q + 1);
-      if (qn < 27) fetch(qn, gi + 1 < G ? gi + 1 : 0, sn, xn);
+      const int qn = gi + 1 < G ? q : qnext;
+      if (qn >= 0) fetch(qn, gi + 1 < G ? gi + 1 : 0, sn, xn);
 #pragma unroll
       for (int c = 0; c < 8; c++) {
         if (sc[c] < 0) continue;
@@ -897,15 +905,34 @@ __global__ __launch_bounds__(64) void k_m2l_tile(VlistArgs v, const float2* __re
         for (int t = 0; t < 8; t++) {
           const int d = ((c >> 2) - (t >> 2) + 1) * 9 + (((c >> 1) & 1) - ((t >> 1) & 1) + 1) * 3 +
                         ((c & 1) - (t & 1) + 1);
-          k1[gi][t] = fmaf(xr, ta[d], k1[gi][t]);
-          k2[gi][t] = fmaf(xi, tb[d], k2[gi][t]);
-          k3[gi][t] = fmaf(xs, ts[d], k3[gi][t]);
+          const int e = M ? 26 - d : d;
+          k1[gi][t] = fmaf(xr, ta[e], k1[gi][t]);
+          k2[gi][t] = fmaf(M ? -xi : xi, tb[e], k2[gi][t]);
+          k3[gi][t] = fmaf(xs, ts[e], k3[gi][t]);
         }
       }
-      if (qn < 27) {
+      if (qn >= 0) {
 #pragma unroll
         for (int c = 0; c < 8; c++) { sc[c] = sn[c]; xc[c] = xn[c]; }
       }
+    }
+  };
+  fetch(0, 0, sc, xc);
+  if constexpr (MIRROR) {
+    // offsets q and 26 - q (= -q) in turn on one load of translation spectra
+    for (int q = 0; q < 13; q++) {
+      load_t(q);
+      sweep(std::false_type{}, q, 26 - q);
+#pragma unroll
+      for (int d = 0; d < 27; d++) ts[d] = ta[d] - tb[d];
+      sweep(std::true_type{}, 26 - q, q + 1 < 13 ? q + 1 : -1);
+    }
+  } else {
+    for (int q = 0; q < 27; q++) {
+      if (q == 13) continue;  // the parent itself: siblings are adjacent
+      load_t(q);
+      const int qn = q + 1 == 13 ? 14 : q + 1;
+      sweep(std::false_type{}, q, qn < 27 ? qn : -1);
     }
   }
   if (!fok) return;
@@ -1169,10 +1196,14 @@ hipError_t launch_hadamard_mixed(const VlistArgs& v, const float2* dspec, const 
     const int ntile = int((nvg + kTileG - 1) / kTileG);
     const int per_xcd = (ntile + 7) / 8;
     const dim3 grid2(unsigned(8 * per_xcd), unsigned((v.H + 63) / 64));
-    if (uint64_t(v.nsrc) * v.nrhs * v.H * 8 <= 0xFFFFFF00ull)
-      k_m2l_tile<true><<<grid2, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm, ntile, per_xcd);
-    else
-      k_m2l_tile<false><<<grid2, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm, ntile, per_xcd);
+    const bool buf = uint64_t(v.nsrc) * v.nrhs * v.H * 8 <= 0xFFFFFF00ull;
+    if (v.mirror) {
+      if (buf) k_m2l_tile<true, true><<<grid2, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm, ntile, per_xcd);
+      else k_m2l_tile<false, true><<<grid2, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm, ntile, per_xcd);
+    } else {
+      if (buf) k_m2l_tile<true, false><<<grid2, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm, ntile, per_xcd);
+      else k_m2l_tile<false, false><<<grid2, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm, ntile, per_xcd);
+    }
   }
   else if (v.had_form == 0) {  // one wave per group, translation spectra read per pair
     // source spectra through a buffer resource when they fit 32-bit offsets

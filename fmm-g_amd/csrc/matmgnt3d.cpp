@@ -160,6 +160,69 @@ void host_rfft3(int n, int howmany, const double* in, double* out) {
   }
 }
 
+// M2L spectra: for each offset slot s (|s|_inf >= 2) the r2c spectrum of
+// K(grid point - source at 2 s) over the n^3 regular grid, [slot][sd*td][H]
+// complex.  nowrap: the grid points with a coordinate index np (displacement
+// -np, the only one without its mirror; no target of the block convolution
+// reads it) hold zero, so that for an even kernel the spectra of opposite
+// offsets are conjugates.
+static void m2l_spectra(int kt, const double* cf, int np, int sd, int td, bool nowrap, std::vector<double>& out) {
+  const int n = 2 * np, G = n * n * n, H = (np + 1) * n * n, sdtd = sd * td;
+  std::vector<double> grid(3 * size_t(G));
+  regular_positions(np, 1.0, grid.data());
+  out.assign(size_t(343) * sdtd * H * 2, 0.0);
+  std::vector<int> offs;
+  for (int i0 = -3; i0 <= 3; i0++)
+    for (int i1 = -3; i1 <= 3; i1++)
+      for (int i2 = -3; i2 <= 3; i2++)
+        if (std::max({std::abs(i0), std::abs(i1), std::abs(i2)}) > 1) offs.push_back(Operators::slot(i0, i1, i2));
+#pragma omp parallel for schedule(dynamic)
+  for (size_t u = 0; u < offs.size(); u++) {
+    int s = offs[u];
+    int i0 = s / 49 - 3, i1 = (s / 7) % 7 - 3, i2 = s % 7 - 3;
+    double src[3] = {i0 * 2.0, i1 * 2.0, i2 * 2.0};
+    std::vector<double> tt(size_t(G) * td * sd), tmp(size_t(G) * sdtd), spec(size_t(H) * sdtd * 2);
+    build_kernel_matrix(kt, cf, 1, src, src, G, grid.data(), tt.data());
+    for (int g = 0; g < G; g++) {
+      const bool wrap = nowrap && (g % n == np || (g / n) % n == np || g / (n * n) == np);
+      for (int i = 0; i < td; i++)
+        for (int j = 0; j < sd; j++)
+          tmp[size_t(g) * sdtd + j * td + i] = wrap ? 0.0 : tt[(size_t(g) * td + i) + size_t(j) * G * td];
+    }
+    host_rfft3(n, sdtd, tmp.data(), spec.data());
+    double* dst = &out[size_t(s) * sdtd * H * 2];
+    for (int f = 0; f < H; f++)
+      for (int c = 0; c < sdtd; c++) {
+        dst[(size_t(c) * H + f) * 2] = spec[(size_t(f) * sdtd + c) * 2];
+        dst[(size_t(c) * H + f) * 2 + 1] = spec[(size_t(f) * sdtd + c) * 2 + 1];
+      }
+  }
+}
+
+std::vector<double> m2l_table_nowrap(const Operators& op) {
+  std::vector<double> t;
+  m2l_spectra(op.kt_mm, op.coefs.data(), op.np, op.sd, op.td, true, t);
+  return t;
+}
+
+bool table_mirror(const std::vector<double>& t, int sdtd, int H) {
+  const size_t per = size_t(sdtd) * H * 2;
+  double big = 0.0, diff = 0.0;
+  for (double x : t) big = std::max(big, std::fabs(x));
+  for (int i0 = -3; i0 <= 3; i0++)
+    for (int i1 = -3; i1 <= 3; i1++)
+      for (int i2 = -3; i2 <= 3; i2++) {
+        if (std::max({std::abs(i0), std::abs(i1), std::abs(i2)}) <= 1) continue;
+        const double* a = t.data() + size_t(Operators::slot(i0, i1, i2)) * per;
+        const double* b = t.data() + size_t(Operators::slot(-i0, -i1, -i2)) * per;
+        for (size_t f = 0; f < per; f += 2) {
+          diff = std::max(diff, std::fabs(a[f] - b[f]));
+          diff = std::max(diff, std::fabs(a[f + 1] + b[f + 1]));
+        }
+      }
+  return diff <= 1e-12 * big;
+}
+
 // ---------------------------------------------------------------------------
 static std::shared_ptr<Operators> build_operators(int kt, const std::vector<double>& coefs, int np) {
   auto op = std::make_shared<Operators>();
@@ -219,35 +282,7 @@ static std::shared_ptr<Operators> build_operators(int kt, const std::vector<doub
       build_kernel_matrix(kt, cf, nde, den.data(), den.data(), ndc, chk.data(), &op->l2l[c * ll]);
     }
   }
-  {  // UpwEqu2DwnChk table :383-410: spectrum of K(grid point - source at 2*idx)
-    const int G = op->G, H = op->H, sdtd = sd * td;
-    std::vector<double> grid(3 * size_t(G));
-    regular_positions(np, 1.0, grid.data());
-    op->m2l.assign(size_t(343) * sdtd * H * 2, 0.0);
-    std::vector<int> offs;
-    for (int i0 = -3; i0 <= 3; i0++)
-      for (int i1 = -3; i1 <= 3; i1++)
-        for (int i2 = -3; i2 <= 3; i2++)
-          if (std::max({std::abs(i0), std::abs(i1), std::abs(i2)}) > 1) offs.push_back(Operators::slot(i0, i1, i2));
-#pragma omp parallel for schedule(dynamic)
-    for (size_t u = 0; u < offs.size(); u++) {
-      int s = offs[u];
-      int i0 = s / 49 - 3, i1 = (s / 7) % 7 - 3, i2 = s % 7 - 3;
-      double src[3] = {i0 * 2.0, i1 * 2.0, i2 * 2.0};
-      std::vector<double> tt(size_t(G) * td * sd), tmp(size_t(G) * sdtd), spec(size_t(H) * sdtd * 2);
-      build_kernel_matrix(kt, cf, 1, src, src, G, grid.data(), tt.data());
-      for (int g = 0; g < G; g++)
-        for (int i = 0; i < td; i++)
-          for (int j = 0; j < sd; j++) tmp[size_t(g) * sdtd + j * td + i] = tt[(size_t(g) * td + i) + size_t(j) * G * td];
-      host_rfft3(n, sdtd, tmp.data(), spec.data());
-      double* dst = &op->m2l[size_t(s) * sdtd * H * 2];
-      for (int f = 0; f < H; f++)
-        for (int c = 0; c < sdtd; c++) {
-          dst[(size_t(c) * H + f) * 2] = spec[(size_t(f) * sdtd + c) * 2];
-          dst[(size_t(c) * H + f) * 2 + 1] = spec[(size_t(f) * sdtd + c) * 2 + 1];
-        }
-    }
-  }
+  m2l_spectra(kt, cf, np, sd, td, false, op->m2l);  // UpwEqu2DwnChk table :383-410
   return op;
 }
 

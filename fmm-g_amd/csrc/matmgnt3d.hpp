@@ -35,6 +35,15 @@ struct Operators {
 std::shared_ptr<const Operators> get_operators(int kt_mm, const std::vector<double>& coefs, int np,
                                                std::string* err);
 
+// The M2L spectra of op recomputed with the kernel grid's wrap entries (a
+// coordinate index np: displacement -np, which no target of the block
+// convolution reads) set to zero; same layout as Operators::m2l, and equal to it
+// in what any evaluate computes.  For an even kernel the spectra of opposite
+// offsets are then conjugates: table_mirror checks that, T(-s) = conj(T(s)) to
+// 1e-12 of the table's largest value.
+std::vector<double> m2l_table_nowrap(const Operators& op);
+bool table_mirror(const std::vector<double>& t, int sdtd, int H);
+
 // matmgnt3d_mpi.cpp:451-499
 int sample_count(int np);
 void sample_positions(int np, double R, double* pos);
