@@ -810,15 +810,11 @@ int FMM3d::upload_state() {
       if ((rc = check(espec_d_.alloc(sizeof(double2) * nt * op.H), "alloc V spectra"))) return rc;
       m2l32_d_.release(); dspec32_d_.release(); espec32_d_.release();
     } else {
-      // scalar kernels: the FP32 table without the kernel grid's wrap entries
-      // (equal in every product an evaluate forms), whose opposite offsets are
-      // conjugates when the kernel is even
-      m2l_mirror_ = false;
-      std::vector<double> nowrap;
-      if (op.sd == 1 && op.td == 1) {
-        nowrap = m2l_table_nowrap(op);
-        m2l_mirror_ = table_mirror(nowrap, 1, op.H);
-      }
+      // the FP32 table without the kernel grid's wrap entries (equal in every
+      // product an evaluate forms) when its opposite offsets are then
+      // conjugates (even kernels: 111, 511; not 301, whose fourth column is odd)
+      const std::vector<double> nowrap = m2l_table_nowrap(op);
+      m2l_mirror_ = table_mirror(nowrap, op.sd * op.td, op.H);
       const std::vector<float> m2l32 = pack_spectra32(m2l_mirror_ ? nowrap : op.m2l, op.np);
       KIFMM_UP(m2l32_d_, m2l32, "upload M2L (FP32)");
       m2l_sym3_ = table_sym3(op.m2l, op.sd, op.td, op.H);

@@ -332,6 +332,7 @@ int FMM3d::evaluateAdjoint(const double* d_trg_in, double* d_src_out, hipStream_
     v.m2l = m2lT_d_.as<double2>();
     m2l32 = m2lT32_d_.as<float2>();
     v.sym3 = m2lT_sym3_ ? 1 : 0;
+    v.mirror = 0;  // the transposed table keeps the wrap entries
     v.deg = deg0_d_.as<double>();
     v.deg_out = deg_d_.as<double>();
   }

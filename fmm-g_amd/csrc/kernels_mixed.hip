@@ -960,7 +960,7 @@ __global__ __launch_bounds__(64) void k_m2l_tile(VlistArgs v, const float2* __re
 // translation value feeds 8 G complex multiply-adds (the per-pair kernel
 // k_hadamard_mx reads one per multiply-add).  Pairs of adjacent children fall
 // on the zero spectra of the table's |offset| <= 1 slots, as in k_m2l_tile.
-template <int SD, int TD, int G, bool BUFX>
+template <int SD, int TD, int G, bool BUFX, bool MIRROR>
 __global__ __launch_bounds__(64) void k_m2l_tile_vec(VlistArgs v, const float2* __restrict__ dspec,
                                                      const float2* __restrict__ m2l, float2* __restrict__ espec,
                                                      float nrm, int ntile, int per_xcd) {
@@ -1012,52 +1012,95 @@ __global__ __launch_bounds__(64) void k_m2l_tile_vec(VlistArgs v, const float2* 
     for (int t = 0; t < 8; t++)
 #pragma unroll
       for (int i = 0; i < TD; i++) acc[gi][t][i] = make_float2(0.f, 0.f);
-  for (int q = 0; q < 27; q++) {
-    if (q == 13) continue;  // the parent itself: siblings are adjacent
+  // colleague offset q: a bit per present (group, source child), and the
+  // sources' spectra of DOF j
+  auto presence = [&](int q) -> unsigned {
     const int qx = q / 9 - 1, qy = (q / 3) % 3 - 1, qz = q % 3 - 1;
     const int hbase = (2 * qx + 2) * 36 + (2 * qy + 2) * 6 + (2 * qz + 2);
-    const int sbase = (2 * qx + 3) * 49 + (2 * qy + 3) * 7 + (2 * qz + 3);
-    // a bit per present (group, source child) of this colleague
     unsigned pres = 0;
 #pragma unroll
     for (int gi = 0; gi < G; gi++)
 #pragma unroll
       for (int c = 0; c < 8; c++)
         if (shal[gi * 216 + hbase + (c >> 2) * 36 + ((c >> 1) & 1) * 6 + (c & 1)] >= 0) pres |= 1u << (gi * 8 + c);
-    pres = __builtin_amdgcn_readfirstlane(pres);
-    if (!pres) continue;  // no source in this colleague
-    for (int j = 0; j < SD; j++) {
-      float2 x[G][8];
+    return __builtin_amdgcn_readfirstlane(pres);
+  };
+  auto load_x = [&](int q, int j, float2 (&x)[G][8]) {
+    const int qx = q / 9 - 1, qy = (q / 3) % 3 - 1, qz = q % 3 - 1;
+    const int hbase = (2 * qx + 2) * 36 + (2 * qy + 2) * 6 + (2 * qz + 2);
 #pragma unroll
-      for (int gi = 0; gi < G; gi++)
+    for (int gi = 0; gi < G; gi++)
 #pragma unroll
-        for (int c = 0; c < 8; c++) {
-          const int sc = __builtin_amdgcn_readfirstlane(
-              shal[gi * 216 + hbase + (c >> 2) * 36 + ((c >> 1) & 1) * 6 + (c & 1)]);
-          x[gi][c] = ldx((sc < 0 ? 0 : sc) * SD + j);
+      for (int c = 0; c < 8; c++) {
+        const int sc = __builtin_amdgcn_readfirstlane(
+            shal[gi * 216 + hbase + (c >> 2) * 36 + ((c >> 1) & 1) * 6 + (c & 1)]);
+        x[gi][c] = ldx((sc < 0 ? 0 : sc) * SD + j);
+      }
+  };
+  // the 27 translation spectra 2q + [-1,1]^3 of DOFs (j, i)
+  auto load_t = [&](int q, int j, int i, float2 (&T)[27]) {
+    const int qx = q / 9 - 1, qy = (q / 3) % 3 - 1, qz = q % 3 - 1;
+    const int sbase = (2 * qx + 3) * 49 + (2 * qy + 3) * 7 + (2 * qz + 3);
+#pragma unroll
+    for (int d = 0; d < 27; d++) {
+      const int sl = sbase + (d / 9 - 1) * 49 + ((d / 3) % 3 - 1) * 7 + (d % 3 - 1);
+      T[d] = ldt(sl * (SD * TD) + j * TD + i);
+    }
+  };
+  // products of target DOF i; mirrored: the offset is the opposite of the one
+  // whose spectra T holds, so the spectrum of 2q + delta is the conjugate of
+  // T[26 - d]
+  auto products = [&](auto mirrored, unsigned pres, const float2 (&x)[G][8], const float2 (&T)[27], int i) {
+    constexpr bool M = decltype(mirrored)::value;
+#pragma unroll
+    for (int gi = 0; gi < G; gi++)
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        if (!((pres >> (gi * 8 + c)) & 1)) continue;
+        const float2 xv = x[gi][c];
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+          const int d = ((c >> 2) - (t >> 2) + 1) * 9 + (((c >> 1) & 1) - ((t >> 1) & 1) + 1) * 3 +
+                        ((c & 1) - (t & 1) + 1);
+          const int e = M ? 26 - d : d;
+          const float tx = T[e].x, ty = M ? -T[e].y : T[e].y;
+          acc[gi][t][i].x = fmaf(tx, xv.x, fmaf(-ty, xv.y, acc[gi][t][i].x));
+          acc[gi][t][i].y = fmaf(tx, xv.y, fmaf(ty, xv.x, acc[gi][t][i].y));
         }
+      }
+  };
+  if constexpr (MIRROR) {
+    // offsets q and 26 - q (= -q) on one load of translation spectra
+    for (int q = 0; q < 13; q++) {
+      const unsigned pq = presence(q), pm = presence(26 - q);
+      if (!(pq | pm)) continue;  // no source in either colleague
+      for (int j = 0; j < SD; j++) {
+        float2 xq[G][8], xm[G][8];
+        if (pq) load_x(q, j, xq);
+        if (pm) load_x(26 - q, j, xm);
 #pragma unroll
-      for (int i = 0; i < TD; i++) {
-        float2 T[27];
-#pragma unroll
-        for (int d = 0; d < 27; d++) {
-          const int sl = sbase + (d / 9 - 1) * 49 + ((d / 3) % 3 - 1) * 7 + (d % 3 - 1);
-          T[d] = ldt(sl * (SD * TD) + j * TD + i);
+        for (int i = 0; i < TD; i++) {
+          float2 T[27];
+          load_t(q, j, i, T);
+          if (pq) products(std::false_type{}, pq, xq, T, i);
+          if (pm) products(std::true_type{}, pm, xm, T, i);
         }
+      }
+    }
+  } else {
+    for (int q = 0; q < 27; q++) {
+      if (q == 13) continue;  // the parent itself: siblings are adjacent
+      const unsigned pres = presence(q);
+      if (!pres) continue;  // no source in this colleague
+      for (int j = 0; j < SD; j++) {
+        float2 x[G][8];
+        load_x(q, j, x);
 #pragma unroll
-        for (int gi = 0; gi < G; gi++)
-#pragma unroll
-          for (int c = 0; c < 8; c++) {
-            if (!((pres >> (gi * 8 + c)) & 1)) continue;
-            const float2 xv = x[gi][c];
-#pragma unroll
-            for (int t = 0; t < 8; t++) {
-              const int d = ((c >> 2) - (t >> 2) + 1) * 9 + (((c >> 1) & 1) - ((t >> 1) & 1) + 1) * 3 +
-                            ((c & 1) - (t & 1) + 1);
-              acc[gi][t][i].x = fmaf(T[d].x, xv.x, fmaf(-T[d].y, xv.y, acc[gi][t][i].x));
-              acc[gi][t][i].y = fmaf(T[d].x, xv.y, fmaf(T[d].y, xv.x, acc[gi][t][i].y));
-            }
-          }
+        for (int i = 0; i < TD; i++) {
+          float2 T[27];
+          load_t(q, j, i, T);
+          products(std::false_type{}, pres, x, T, i);
+        }
       }
     }
   }
@@ -1230,9 +1273,12 @@ hipError_t launch_hadamard_mixed(const VlistArgs& v, const float2* dspec, const 
     const dim3 grid2(unsigned(8 * per_xcd), unsigned((v.H + 63) / 64));
     // source spectra through a buffer resource when they fit 32-bit offsets
     const bool bufx = uint64_t(v.nsrc) * v.nrhs * v.sd * v.H * 8 <= 0xFFFFFF00ull;
+#define KIFMM_TILE_GM(SD_, TD_, G_, M_)                                                                          \
+  if (bufx) k_m2l_tile_vec<SD_, TD_, G_, true, M_><<<grid2, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm, ntile, per_xcd); \
+  else k_m2l_tile_vec<SD_, TD_, G_, false, M_><<<grid2, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm, ntile, per_xcd);
 #define KIFMM_TILE_G(SD_, TD_, G_)                                                                          \
-  if (bufx) k_m2l_tile_vec<SD_, TD_, G_, true><<<grid2, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm, ntile, per_xcd); \
-  else k_m2l_tile_vec<SD_, TD_, G_, false><<<grid2, dim3(64), 0, s>>>(v, dspec, m2l, espec, nrm, ntile, per_xcd);
+  if (v.mirror) { KIFMM_TILE_GM(SD_, TD_, G_, true) }                                                       \
+  else { KIFMM_TILE_GM(SD_, TD_, G_, false) }
 #define KIFMM_TILE(SD_, TD_)                                                                                \
   if (v.sd == SD_ && v.td == TD_) {                                                                         \
     if (G == 1) { KIFMM_TILE_G(SD_, TD_, 1) }                                                               \
@@ -1244,6 +1290,7 @@ hipError_t launch_hadamard_mixed(const VlistArgs& v, const float2* dspec, const 
     KIFMM_TILE(3, 4)
 #undef KIFMM_TILE
 #undef KIFMM_TILE_G
+#undef KIFMM_TILE_GM
     return hipErrorInvalidValue;
   }
   return hipGetLastError();
