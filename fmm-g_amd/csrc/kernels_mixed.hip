@@ -797,6 +797,12 @@ __global__ __launch_bounds__(64) void k_hadamard_mx(VlistArgs v, const float2* _
 // the scalar offset and the lane's frequency in the vector offset, so the
 // loads need no vector address arithmetic (both tables must fit in 32-bit byte
 // offsets).
+//
+// MIRROR: the table's spectra of opposite slots are conjugates (an even kernel
+// on a grid without its unpaired wrap displacement), so offset -q (= 26 - q)
+// runs right after q on the same registers: slot -2q + delta reads register
+// 26 - d with the imaginary part negated (A2 takes -d, A3 takes a - b), and each
+// pair of offsets loads 27 translation spectra instead of 54.
 constexpr int kTileG = 5;
 
 template <bool BUF, bool MIRROR>
