@@ -199,6 +199,9 @@ struct VlistArgs {
   // conj(T(s)) (scalar even kernel, table without the kernel grid's wrap
   // entries): the mixed halo sweep serves offsets q and -q from one load
   int mirror = 0;
+  // the targets' check values are zero when the inverse transform runs (and
+  // no other writer comes first): it stores instead of reading and adding
+  int dc_store = 0;
 };
 
 // Adjoint near field (source-centric): chunk = <= kNearChunk sources of a

@@ -1258,6 +1258,7 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
   vlist_args(v, vfwd_);
   v.UE = ue_src; v.sz_ue = op.dat(UE);
   v.DC = dc_d_.as<double>(); v.sz_dc = op.dat(DC);
+  v.dc_store = 1;  // DC was zeroed above and the V list writes it first (S2L adds after)
   v.nrhs = R;
   v.nstride = N;
   if ((rc = run_vlist(v, s, true))) return rc;

@@ -747,9 +747,13 @@ __global__ __launch_bounds__(64) void k_fft_inv(VlistArgs v, const C2* espec) {
       int qc[NP];
       double old[NP];
 #pragma unroll
-      for (int c = 0; c < NP; c++) {
-        qc[c] = qof[ab * NP + c];
-        old[c] = dc[(qc[c] < 0 ? 0 : qc[c]) * v.td + i];
+      for (int c = 0; c < NP; c++) qc[c] = qof[ab * NP + c];
+      if (v.dc_store) {  // 0 + x: the same sums as reading the zeroed values
+#pragma unroll
+        for (int c = 0; c < NP; c++) old[c] = 0.0;
+      } else {
+#pragma unroll
+        for (int c = 0; c < NP; c++) old[c] = dc[(qc[c] < 0 ? 0 : qc[c]) * v.td + i];
       }
 #pragma unroll
       for (int c = 0; c < NP; c++) {
