@@ -241,6 +241,10 @@ hipError_t launch_surface_adjoint(int km, const SurfTArgs& a, const KParams<doub
 hipError_t launch_gather(const double* in, const int* perm, double* out, int64_t n, int dof, hipStream_t s);
 hipError_t launch_scatter(const double* in, const int* perm, double* out, int64_t n, int dof, hipStream_t s);
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s);
+// Zero rows[k] + r * nstride (k < nrows, r < nrhs) of a row-major array of rows
+// of len doubles.
+hipError_t launch_zero_rows(double* base, const int* rows, int nrows, int len, int nrhs, int64_t nstride,
+                            hipStream_t s);
 // nrhs density vectors [r][k][dof] (caller's order) -> leaf order [sinv[k]][r][dof]
 hipError_t launch_load_rhs(const double* in, const int* sinv, double* out, int64_t n, int dof, int nrhs,
                            hipStream_t s);

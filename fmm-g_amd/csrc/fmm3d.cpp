@@ -535,6 +535,20 @@ int FMM3d::build_tasks() {
   work_.n_vsrc = double(vfwd_.src_node.size());
   work_.n_vtrg = double(vfwd_.trg_node.size());
   work_.n_vgrp = double(vfwd_.ngrp());
+  // rows the mixed evaluate reads without writing them first, zeroed at its
+  // start instead of the whole buffers: UE of the nodes without sources (the
+  // fused M2M reads all 8 children of a parent), DC of the evaluator nodes of
+  // depth >= 2 that are not V-list targets (the V-list inverse stores the others;
+  // S2L adds after it and the downward step reads them)
+  ue_zero_h_.clear(); dc_zero_h_.clear();
+  {
+    std::vector<char> vt(N, 0);
+    for (int B : vfwd_.trg_node) vt[B] = 1;
+    for (int B = 0; B < N; B++) {
+      if (t.depth[B] >= 1 && !(t.tag[B] & TAG_CBTR)) ue_zero_h_.push_back(B);
+      if (t.depth[B] >= 2 && (t.tag[B] & TAG_EVTR) && !vt[B]) dc_zero_h_.push_back(B);
+    }
+  }
   adj_ready_ = false;
   return 0;
 }
@@ -702,6 +716,8 @@ int FMM3d::upload_state() {
   KIFMM_UP(near_items_d_, near_items_h_, "upload near items");
   KIFMM_UP(s2m_tasks_d_, s2m_tasks_h_, "upload S2M tasks");
   KIFMM_UP(s2m_leaf_d_, s2m_leaf_h_, "upload S2M tasks");
+  KIFMM_UP(ue_zero_d_, ue_zero_h_, "upload zero rows");
+  KIFMM_UP(dc_zero_d_, dc_zero_h_, "upload zero rows");
   KIFMM_UP(s2l_tasks_d_, s2l_tasks_h_, "upload S2L tasks");
   KIFMM_UP(surf_items_d_, surf_items_h_, "upload surface items");
   KIFMM_UP(lvl_idx_d_, lvl_idx_h_, "upload level columns");
@@ -1154,7 +1170,12 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
   mark(PH_INI);
   // zero the work vectors (:425-438)
   const int64_t NR = int64_t(N) * R;  // node rows over the right-hand sides
-  if (!ue_in && (rc = check(launch_fill(ue_d_.as<double>(), NR * op.dat(UE), 0.0, s), "zero UE"))) return rc;
+  // mixed path: only the rows read before any write (the rest is stored first)
+  if (!ue_in && (rc = check(mixed ? launch_zero_rows(ue_d_.as<double>(), ue_zero_d_.as<int>(), int(ue_zero_h_.size()),
+                                                     op.dat(UE), R, N, s)
+                                  : launch_fill(ue_d_.as<double>(), NR * op.dat(UE), 0.0, s),
+                            "zero UE")))
+    return rc;
   // mixed path: UC is read only for the contributor leaves, whose S2M task
   // stores it whole (the fused up pass forms no UC for parents)
   if (!mixed && (rc = check(launch_fill(uc_d_.as<double>(), NR * op.dat(UC), 0.0, s), "zero UC")))
@@ -1163,7 +1184,11 @@ int FMM3d::evaluate_impl(const double* d_src_den, double* d_trg_val, hipStream_t
   // >= 2, and only those rows are read (L2L, L2T); the mixed path skips the fill
   if (!mixed && (rc = check(launch_fill(de_d_.as<double>(), NR * op.dat(DE), 0.0, s), "zero DE")))
     return rc;
-  if ((rc = check(launch_fill(dc_d_.as<double>(), NR * op.dat(DC), 0.0, s), "zero DC"))) return rc;
+  if ((rc = check(mixed ? launch_zero_rows(dc_d_.as<double>(), dc_zero_d_.as<int>(), int(dc_zero_h_.size()), op.dat(DC),
+                                           R, N, s)
+                        : launch_fill(dc_d_.as<double>(), NR * op.dat(DC), 0.0, s),
+                  "zero DC")))
+    return rc;
   if ((rc = check(launch_load_rhs(d_src_den, sinv_d_.as<int>(), sden_d_.as<double>(), ns_, SD, R, s),
                   "load densities")))
     return rc;

@@ -295,6 +295,8 @@ class FMM3d {
   DeviceBuffer ue_d_, uc_d_, dc_d_, de_d_;
   DeviceBuffer near_chunks_mx_d_;
   DeviceBuffer near_chunks_d_, near_items_d_, s2m_tasks_d_, s2l_tasks_d_, surf_items_d_, s2m_leaf_d_;
+  std::vector<int> ue_zero_h_, dc_zero_h_;  // mixed evaluate: UE / DC rows zeroed at its start
+  DeviceBuffer ue_zero_d_, dc_zero_d_;
   DeviceBuffer lvl_idx_d_;
   DeviceBuffer sam_d_[4], shell_d_, twiddle_d_, deg_d_, ones_d_, scales_d_;
   DeviceBuffer pup_d_, pdn_d_, m2m_d_, l2l_d_, m2l_d_;

@@ -82,6 +82,39 @@ hipError_t launch_store_rhs(const double* in, const int* perm, double* out, int6
   k_store_rhs<<<dim3(unsigned((n * nrhs + 255) / 256)), dim3(256), 0, s>>>(in, perm, out, n, dof, nrhs);
   return hipGetLastError();
 }
+// One thread per (row, right-hand side, column); 16-byte stores when the row
+// length is even (rows of 8-byte-aligned length: 152, 296, ...).
+__global__ void k_zero_rows(double* base, const int* rows, int nrows, int len, int nrhs, int64_t nstride) {
+  const int half = len / 2;
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= int64_t(nrows) * nrhs * half) return;
+  const int col = int(i % half);
+  const int64_t kr = i / half;
+  const int k = int(kr % nrows), r = int(kr / nrows);
+  double* row = base + (int64_t(rows[k]) + r * nstride) * len;
+  reinterpret_cast<double2*>(row)[col] = make_double2(0.0, 0.0);
+}
+__global__ void k_zero_rows_odd(double* base, const int* rows, int nrows, int len, int nrhs, int64_t nstride) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= int64_t(nrows) * nrhs * len) return;
+  const int col = int(i % len);
+  const int64_t kr = i / len;
+  const int k = int(kr % nrows), r = int(kr / nrows);
+  base[(int64_t(rows[k]) + r * nstride) * len + col] = 0.0;
+}
+hipError_t launch_zero_rows(double* base, const int* rows, int nrows, int len, int nrhs, int64_t nstride,
+                            hipStream_t s) {
+  if (nrows <= 0 || nrhs <= 0 || len <= 0) return hipSuccess;
+  if (len % 2 == 0) {
+    const int64_t n = int64_t(nrows) * nrhs * (len / 2);
+    k_zero_rows<<<dim3(unsigned((n + 255) / 256)), dim3(256), 0, s>>>(base, rows, nrows, len, nrhs, nstride);
+  } else {
+    const int64_t n = int64_t(nrows) * nrhs * len;
+    k_zero_rows_odd<<<dim3(unsigned((n + 255) / 256)), dim3(256), 0, s>>>(base, rows, nrows, len, nrhs, nstride);
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   int64_t blocks = (n + 255) / 256;
