@@ -241,11 +241,15 @@ class Runner:
             f.evaluate_device(d_den, d_val)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / steps
-        # the same pass replayed from a captured hipGraph on a stream of its own
+        # the same pass replayed from a captured hipGraph on a stream of its own;
+        # the first call captures and launches, then the same number of untimed
+        # replays as the direct loop's warmup (the kernels run a few percent
+        # slower in the first replays after the capture pause)
         st = torch.cuda.Stream(device=self.dev)
         f.set_graph(True)
         with torch.cuda.stream(st):
-            f.evaluate_device(d_den, d_val, stream=st.cuda_stream)
+            for _ in range(1 + warmup):
+                f.evaluate_device(d_den, d_val, stream=st.cuda_stream)
             st.synchronize()
             t0 = time.perf_counter()
             for _ in range(steps):
